@@ -1,0 +1,173 @@
+/*
+ * dgk.h -- C ABI of the MI355X DG advection library (libdgk.so).
+ *
+ * The library is a drop-in for the reference's operator selection
+ * (DGOptrKernel_OptType = GENERAL | OPT1, main.f90:101-107 ->
+ * dg_optr_kernel_setup, mod_dg_optr_kernel.f90:45-58) one level up: it takes
+ * over the whole-mesh tendency advect3d_eq_cal_tend
+ * (mod_advect3d_eq.f90:41-85) together with the stage loop's update_halo and
+ * RK update (main.f90:51-75), with the state resident in device memory.
+ *
+ * ABI rules
+ *   * Plain C: pointers and sizes only, no ownership transfer.  Arrays are the
+ *     reference's Fortran arrays passed by reference (column-major, real64,
+ *     default integer = int32, 1-based flat indices in VMapM/VMapP/
+ *     halo_src_map).  The library copies what it needs; the caller keeps its
+ *     host memory.
+ *   * Every entry point returns an int status (DGK_OK = 0).  On failure the
+ *     message of the calling thread's last error is in dgk_last_error().
+ *   * A solver handle is single-thread-affine and owns one HIP stream; its
+ *     calls are ordered on that stream.  Calls that return host data
+ *     synchronize the stream.
+ *   * The device path has no CPU fallback: when no gfx950 device is present,
+ *     dgk_create* fail with DGK_ERR_DEVICE.
+ */
+#ifndef DGK_H
+#define DGK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  DGK_OK = 0,
+  DGK_ERR_ARG = 1,         /* null pointer, size or range error              */
+  DGK_ERR_UNSUPPORTED = 2, /* valid for the reference, outside the GPU path  */
+  DGK_ERR_DEVICE = 3,      /* HIP runtime error or no usable device          */
+  DGK_ERR_NOMEM = 4,       /* device memory exhausted / over budget          */
+  DGK_ERR_IO = 5,          /* operator file missing or malformed             */
+  DGK_ERR_STATE = 6        /* call out of order                              */
+};
+
+/* ---- operator evaluation types -----------------------------------------
+ * Replaces the string -> eval_typeid switch of dg_optr_kernel_setup
+ * (mod_dg_optr_kernel.f90:45-58).  "GENERAL" and "OPT1" are recognised (they
+ * name the reference CPU kernels and are rejected by dgk_create with
+ * DGK_ERR_UNSUPPORTED); "HIP" selects the FP64 device path and "HIP_MP" the
+ * mixed-precision device path.  Unknown names are an error (DGK_ERR_ARG),
+ * where the reference prints and continues (mod_dg_optr_kernel.f90:54-55). */
+enum { DGK_EVAL_GENERAL = 1, DGK_EVAL_OPT1 = 2, DGK_EVAL_HIP = 3, DGK_EVAL_HIP_MP = 4 };
+
+/* ---- field ids ---------------------------------------------------------- */
+enum { DGK_FIELD_Q = 0, DGK_FIELD_U = 1, DGK_FIELD_V = 2, DGK_FIELD_W = 3,
+       DGK_FIELD_DQDT = 4 };
+
+typedef struct dgk_solver dgk_solver;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *dgk_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int dgk_version(void);
+/* Number of usable HIP devices (0 on a host without one). */
+int dgk_device_count(int *count);
+
+/* Name -> evaluation type id (mod_dg_optr_kernel.f90:45-58). */
+int dgk_evaltype_id(const char *name, int *id);
+
+/* ---- element and operators ------------------------------------------- */
+/* Nq = p+1, Np = Nq^3, Nfp = Nq^2, NfpTot = 6*Nfp (mod_mesh.f90:236-241). */
+int dgk_element_sizes(int p, int *Nq, int *Np, int *Nfp, int *NfpTot);
+/* Read operator_data/p{p}.dat from dir: x1D(Nq), D1D(Nq,Nq),
+ * Lift_mat(Nq,Nq,Nq,6), column-major (element_init, mod_mesh.f90:253-261). */
+int dgk_load_operators(const char *dir, int p, double *x1D, double *D1D,
+                       double *Lift_mat);
+
+/* ---- structured periodic mesh (mesh_setup, mod_mesh.f90:74-193) ------- */
+/* Ne = NeX*NeY*NeZ, Nhalo = 2(NeY*NeZ + NeX*NeZ + NeX*NeY), NeA = Ne + Nhalo,
+ * NhaloNode = Nfp*Nhalo (mod_mesh.f90:96-100). */
+int dgk_mesh_sizes(int NeX, int NeY, int NeZ, int p, int64_t *Ne,
+                   int64_t *NeA, int64_t *NhaloNode);
+/* Fill the reference's mesh arrays for the periodic box [0,dom_x]x[0,dom_y]x
+ * [0,dom_z] (mesh_setup, mod_mesh.f90:74-193; set_element_position :332-353;
+ * set_face_geometry :356-397; eval_neighbor_element :407-483).  Shapes:
+ * VMapM, VMapP (NfpTot,Ne); halo_src_map (NhaloNode); normal_fn (NfpTot,Ne,3);
+ * Escale (Np,Ne,3); Fscale (NfpTot,Ne); pos_en (Np,Ne,3) (may be NULL). */
+int dgk_mesh_build(int NeX, int NeY, int NeZ, int p, const double *x1D,
+                   double dom_x, double dom_y, double dom_z, int *VMapM,
+                   int *VMapP, int *halo_src_map, double *normal_fn,
+                   double *Escale, double *Fscale, double *pos_en);
+
+/* ---- solver ------------------------------------------------------------ */
+/* Create a device solver from the reference's arrays: the inputs of
+ * advect3d_eq_cal_tend (mod_advect3d_eq.f90:41-66) plus the halo map of
+ * update_halo (mod_mesh.f90:209-221).  The connectivity and geometry are
+ * compressed at creation (per-element neighbour table, deduplicated per-face
+ * geometry); arrays the GPU path cannot represent (non-affine geometry, or
+ * VMapM not addressing the element's own Fmask nodes) give
+ * DGK_ERR_UNSUPPORTED.  Lift_mat is used as stored by "HIP"; "HIP_MP" uses
+ * its separable 1D form and rejects a Lift_mat that is not separable to
+ * 1e-10 relative. */
+int dgk_create(int evaltype, int Nq, int64_t Ne, int64_t NeA,
+               int64_t NhaloNode, const double *D1D, const double *Lift_mat,
+               const int *VMapM, const int *VMapP, const int *halo_src_map,
+               const double *normal_fn, const double *Escale,
+               const double *Fscale, dgk_solver **out);
+/* Same solver for the structured periodic box, built without materialising
+ * the per-node arrays (operators read from operator_dir/p{p}.dat). */
+int dgk_create_structured(int evaltype, int NeX, int NeY, int NeZ, int p,
+                          const char *operator_dir, double dom_x,
+                          double dom_y, double dom_z, dgk_solver **out);
+int dgk_destroy(dgk_solver *s);
+
+/* Sizes of a solver: Nq, Ne, NeA, NhaloNode. */
+int dgk_solver_info(const dgk_solver *s, int *Nq, int64_t *Ne, int64_t *NeA,
+                    int64_t *NhaloNode);
+/* Device bytes held by the solver. */
+int dgk_device_bytes(const dgk_solver *s, int64_t *bytes);
+
+/* Upload a field, n = Np*Ne (owned part) or Np*NeA (the halo tail is
+ * ignored: it is derived from the owned part).  Setting U, V or W refreshes
+ * their halos once, as main.f90:160-162 does. */
+int dgk_set_field(dgk_solver *s, int which, const double *f, int64_t n);
+/* Download a field, n = Np*Ne or Np*NeA.  With n = Np*NeA the halo tail holds
+ * what the last dgk_update_halo wrote (zeros before any). */
+int dgk_get_field(dgk_solver *s, int which, double *f, int64_t n);
+
+/* update_halo (mod_mesh.f90:209-221) on the device copy of a field. */
+int dgk_update_halo(dgk_solver *s, int which);
+/* q0 <- q over the owned elements (main.f90:53-56). */
+int dgk_copy_q0(dgk_solver *s);
+/* cal_elembnd_flux (mod_advect3d_eq.f90:89-142) of the current q into the
+ * solver's flux buffer (NfpTot,Ne); copied to flux_host when not NULL. */
+int dgk_cal_elembnd_flux(dgk_solver *s, double *flux_host);
+/* cal_dqdt (mod_advect3d_eq.f90:146-201) from the flux buffer filled by
+ * dgk_cal_elembnd_flux (or uploaded from flux_host when not NULL) into the
+ * device dqdt. */
+int dgk_cal_dqdt(dgk_solver *s, const double *flux_host);
+/* advect3d_eq_cal_tend (mod_advect3d_eq.f90:41-85), fused flux + volume +
+ * Lift in one kernel, into the device dqdt; copied to dqdt_host (Np*Ne) when
+ * not NULL.  Reads the halo through the folded neighbour map, so it needs no
+ * prior dgk_update_halo. */
+int dgk_cal_tend(dgk_solver *s, double *dqdt_host);
+/* RK stage update q = a_s q0 + b_s (q + dt dqdt), stage = 1..3
+ * (main.f90:69-73, coefficients mod_common.f90:28-30). */
+int dgk_rk_update(dgk_solver *s, int stage, double dt);
+/* nstep full SSP-RK3 steps on the device (main.f90:51-75), each stage one
+ * fused kernel (flux + volume + Lift + RK update, halo folded). */
+int dgk_step(dgk_solver *s, double dt, int nstep);
+/* min/max of the owned q (main.f90:76-79). */
+int dgk_minmax(dgk_solver *s, double *qmin, double *qmax);
+/* Wait for the solver's stream. */
+int dgk_synchronize(dgk_solver *s);
+
+/* ---- timers (mod_common.f90:33-71, names of main.f90:175-177 and
+ * mod_advect3d_eq.f90:34-35) ---------------------------------------------
+ * seconds[0] Main (host wall time inside dgk_step),
+ * seconds[1] Cal_tend (device time of tendency kernels),
+ * seconds[2] Element boundary flux (device, dgk_cal_elembnd_flux),
+ * seconds[3] Volume derivate + surface lift (device, dgk_cal_dqdt),
+ * seconds[4] fused stage kernels (device, dgk_step),
+ * counts[0..4] number of timed intervals.  Device times are collected only
+ * while dgk_set_kernel_timing(s, 1) is on (HIP events around each launch). */
+int dgk_set_kernel_timing(dgk_solver *s, int on);
+int dgk_timer_report(dgk_solver *s, double *seconds, int64_t *counts);
+int dgk_timer_reset(dgk_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGK_H */

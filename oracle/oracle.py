@@ -1,0 +1,214 @@
+"""Python access to the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+* `Oracle*` functions wrap liboracle.so, the C restatement of the reference
+  GENERAL path (oracle.c; built by `make` in this directory).
+* `run_reference` drives _ref/ref_harness, the reference Fortran modules
+  compiled from their sources by `make ref` (in the build container; the
+  binary travels with the tree).
+
+Only tests/, the smoke() check and bench.py's cpu_baseline leg use this
+module, and only as the checker.  Array conventions follow the dgk binding:
+numpy axes are the reverse of the reference's Fortran axes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import resource
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "liboracle.so"
+REF_BIN = ORACLE_DIR / "_ref" / "ref_harness"
+OPERATOR_PARENT = ORACLE_DIR.parent / "scale-dg-kernel-extraction_amd"
+
+GENERAL, OPT1 = 1, 2
+_P, _I, _L, _D = C.c_void_p, C.c_int, C.c_long, C.c_double
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        h = C.CDLL(str(LIB_PATH))
+        sig = {
+            "orc_load_operators": (_I, [C.c_char_p, _I, _P, _P, _P]),
+            "orc_fmask": (None, [_I, _P]),
+            "orc_mesh_build": (None, [_I, _I, _I, _I, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
+            "orc_update_halo": (None, [_P, _L, _L, _L, _P]),
+            "orc_elembnd_flux": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _L, _L]),
+            "orc_divlike": (None, [_P, _P, _P, _P, _P, _P, _P, _I, _I]),
+            "orc_lift": (None, [_P, _P, _P, _I]),
+            "orc_cal_dqdt": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _L, _I]),
+            "orc_cal_tend": (None, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _L, _P, _I]),
+            "orc_rk_update": (None, [_P, _P, _P, _D, _I, _L, _L]),
+            "orc_run": (_D, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _L, _L, _L, _D,
+                             _I, _I, _P]),
+            "orc_u01": (_D, [C.c_uint64, C.c_uint64]),
+            "orc_num_threads": (_I, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def load_operators(p: int, directory=OPERATOR_PARENT / "operator_data"):
+    Nq = p + 1
+    x = np.zeros(Nq)
+    D = np.zeros((Nq, Nq))
+    L = np.zeros((6, Nq, Nq, Nq))
+    rc = lib().orc_load_operators(str(Path(directory) / f"p{p}.dat").encode(), Nq,
+                                  _p(x), _p(D), _p(L))
+    if rc:
+        raise IOError(f"orc_load_operators rc={rc}")
+    return x, D, L
+
+
+def fmask(Nq: int) -> np.ndarray:
+    """Fmask as numpy (6, Nfp), 1-based (Fortran Fmask(Nfp,6))."""
+    a = np.zeros((6, Nq * Nq), np.int32)
+    lib().orc_fmask(Nq, _p(a))
+    return a
+
+
+class Mesh:
+    """The reference mesh arrays built by the oracle (mesh_setup restated)."""
+
+    def __init__(self, NeX, NeY, NeZ, p, dom=(1.0, 1.0, 1.0)):
+        self.NeX, self.NeY, self.NeZ, self.p = NeX, NeY, NeZ, p
+        self.Nq = Nq = p + 1
+        self.Np, self.Nfp, self.NfpTot = Nq ** 3, Nq ** 2, 6 * Nq ** 2
+        self.Ne = NeX * NeY * NeZ
+        self.Nhalo = 2 * (NeY * NeZ + NeX * NeZ + NeX * NeY)
+        self.NeA = self.Ne + self.Nhalo
+        self.NhaloNode = self.Nfp * self.Nhalo
+        self.x1D, self.D, self.Lift = load_operators(p)
+        Ne, NfpTot, Np = self.Ne, self.NfpTot, self.Np
+        self.VMapM = np.zeros((Ne, NfpTot), np.int32)
+        self.VMapP = np.zeros((Ne, NfpTot), np.int32)
+        self.halo_src_map = np.zeros(self.NhaloNode, np.int32)
+        self.normal_fn = np.zeros((3, Ne, NfpTot))
+        self.Escale = np.zeros((3, Ne, Np))
+        self.Fscale = np.zeros((Ne, NfpTot))
+        self.pos_en = np.zeros((3, Ne, Np))
+        lib().orc_mesh_build(NeX, NeY, NeZ, Nq, _p(self.x1D), dom[0], dom[1], dom[2],
+                             _p(self.VMapM), _p(self.VMapP), _p(self.halo_src_map),
+                             _p(self.normal_fn), _p(self.Escale), _p(self.Fscale),
+                             _p(self.pos_en))
+
+    def field(self, owned: np.ndarray) -> np.ndarray:
+        """(NeA, Np) field with the owned part set and the halo tail refreshed."""
+        f = np.zeros((self.NeA, self.Np))
+        f[: self.Ne] = owned
+        self.update_halo(f)
+        return f
+
+    def update_halo(self, f: np.ndarray) -> None:
+        lib().orc_update_halo(_p(f), self.Np, self.Ne, self.NhaloNode, _p(self.halo_src_map))
+
+    def elembnd_flux(self, q, u, v, w) -> np.ndarray:
+        fl = np.zeros((self.Ne, self.NfpTot))
+        lib().orc_elembnd_flux(_p(fl), _p(q), _p(u), _p(v), _p(w), _p(self.VMapM),
+                               _p(self.VMapP), _p(self.normal_fn), _p(self.Fscale),
+                               self.NfpTot, self.Ne)
+        return fl
+
+    def cal_dqdt(self, q, u, v, w, flux, variant=GENERAL) -> np.ndarray:
+        out = np.zeros((self.NeA, self.Np))
+        lib().orc_cal_dqdt(_p(out), _p(q), _p(u), _p(v), _p(w), _p(np.ascontiguousarray(flux)),
+                           _p(self.D), _p(self.Lift), _p(self.Escale), self.Nq, self.Ne, variant)
+        return out[: self.Ne]
+
+    def cal_tend(self, q, u, v, w, variant=GENERAL) -> np.ndarray:
+        """advect3d_eq_cal_tend; q,u,v,w are (NeA, Np) with halos filled."""
+        out = np.zeros((self.NeA, self.Np))
+        lib().orc_cal_tend(_p(out), _p(q), _p(u), _p(v), _p(w), _p(self.D), _p(self.Lift),
+                           _p(self.VMapM), _p(self.VMapP), _p(self.normal_fn), _p(self.Escale),
+                           _p(self.Fscale), self.Nq, self.Ne, None, variant)
+        return out[: self.Ne]
+
+    def run(self, q_owned, u_owned, v_owned, w_owned, dt, nstep, variant=GENERAL,
+            trace=False):
+        """The reference time loop; returns (q owned, minmax trace or None, seconds)."""
+        q = self.field(q_owned)
+        u, v, w = self.field(u_owned), self.field(v_owned), self.field(w_owned)
+        mm = np.zeros((max(nstep, 1), 2)) if trace else None
+        t = lib().orc_run(_p(q), _p(u), _p(v), _p(w), _p(self.D), _p(self.Lift), _p(self.VMapM),
+                          _p(self.VMapP), _p(self.halo_src_map), _p(self.normal_fn),
+                          _p(self.Escale), _p(self.Fscale), self.Nq, self.Ne, self.NeA,
+                          self.NhaloNode, float(dt), int(nstep), variant, _p(mm))
+        return q[: self.Ne].copy(), (mm[:nstep] if trace else None), t
+
+
+def u01(seed: int, counter: int) -> float:
+    return lib().orc_u01(seed, counter)
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# --------------------------------------------------------------------------
+# Reference Fortran harness
+# --------------------------------------------------------------------------
+def reference_available() -> bool:
+    return REF_BIN.exists()
+
+
+def _unlimited_stack():
+    # advect3d_eq_cal_tend's automatic array ebnd_flux(NfpTot,Ne) lives on the
+    # stack (mod_advect3d_eq.f90:66): large meshes need an unlimited stack.
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
+def run_reference(mode, NeX, NeY, NeZ, p, opttype, nstep, dt, q, u, v, w,
+                  threads=None, timeout=3600):
+    """Run the reference modules on (q, u, v, w) (each numpy (Ne, Np)).
+
+    mode 'run'  -> (q_final (Ne,Np), minmax (nstep,2), loop_seconds)
+    mode 'tend' -> (dqdt (Ne,Np), None, seconds)
+    mode 'time' -> (None, None, loop_seconds)
+    """
+    if not reference_available():
+        raise RuntimeError(f"{REF_BIN} missing: run `make -C oracle ref`")
+    Ne, Np = NeX * NeY * NeZ, (p + 1) ** 3
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
+        with open(fin, "wb") as fh:
+            for a in (q, u, v, w):
+                np.ascontiguousarray(a, dtype=np.float64).reshape(Ne, Np).tofile(fh)
+        env = dict(os.environ)
+        if threads:
+            env["OMP_NUM_THREADS"] = str(threads)
+        cmd = [str(REF_BIN), mode, str(NeX), str(NeY), str(NeZ), str(p), opttype,
+               str(nstep), repr(float(dt)), str(fin), str(fout)]
+        r = subprocess.run(cmd, cwd=str(OPERATOR_PARENT), env=env, capture_output=True,
+                           text=True, timeout=timeout, preexec_fn=_unlimited_stack)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_harness failed ({r.returncode}): {r.stdout}\n{r.stderr}")
+        m = re.search(r"LOOP_SECONDS\s+(\S+)", r.stdout)
+        secs = float(m.group(1)) if m else float("nan")
+        if mode == "time":
+            return None, None, secs
+        raw = np.fromfile(fout, dtype=np.float64)
+        qf = raw[: Ne * Np].reshape(Ne, Np).copy()
+        mm = raw[Ne * Np:].reshape(-1, 2).copy() if mode == "run" else None
+        return qf, mm, secs

@@ -1,0 +1,603 @@
+// C-ABI entry points of libdgk: solver lifetime, field transfer, the
+// piecewise reference stage loop (update_halo / cal_elembnd_flux / cal_dqdt /
+// RK update) and the fused device time loop.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dgk_internal.hpp"
+
+namespace dgk {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+// SSP-RK3 Shu-Osher coefficients (mod_common.f90:28-30).
+constexpr double kRkA[3] = {0.0, 0.75, 1.0 / 3.0};
+constexpr double kRkB[3] = {1.0, 0.25, 2.0 / 3.0};
+
+// Timer slots (names of main.f90:175-177 and mod_advect3d_eq.f90:34-35).
+enum { T_MAIN = 0, T_CALTEND = 1, T_FLUX = 2, T_DQDT = 3, T_STAGE = 4, T_N = 5 };
+
+}  // namespace dgk
+
+using namespace dgk;
+
+struct dgk_solver {
+  int evaltype = 0;
+  int device = 0;
+  int Nq = 0, Np = 0, Nfp = 0, NfpTot = 0;
+  int64_t Ne = 0, NeA = 0, NhaloNode = 0;
+  int64_t nfield = 0;  // Np*Ne + NhaloNode: owned part + halo tail
+  hipStream_t stream = nullptr;
+  double *q[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
+  double *u = nullptr, *v = nullptr, *w = nullptr;
+  double *q0 = nullptr, *dqdt = nullptr, *flux = nullptr;
+  bool flux_valid = false, dqdt_valid = false, q0_valid = false;
+  float *u32 = nullptr, *v32 = nullptr, *w32 = nullptr;
+  float *D32 = nullptr, *ell32 = nullptr;
+  int32_t *nbr = nullptr, *vmapP = nullptr, *halo = nullptr;
+  double *geo = nullptr, *D = nullptr, *Lift = nullptr;
+  uint32_t *geoid = nullptr;
+  double *mm = nullptr;  // min/max partials + result
+  bool set[4] = {false, false, false, false};
+  int64_t bytes = 0;
+  std::vector<void *> allocs;
+  // timers
+  bool kernel_timing = false;
+  double sec[T_N] = {0, 0, 0, 0, 0};
+  int64_t cnt[T_N] = {0, 0, 0, 0, 0};
+  struct Pending { int slot; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(dgk_solver *s, T **p, int64_t n) {
+  *p = nullptr;
+  if (n <= 0) return DGK_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(DGK_ERR_NOMEM, "hipMalloc of " + std::to_string(sizeof(T) * n) +
+                                        " bytes failed: " + hipGetErrorString(e));
+  }
+  s->allocs.push_back(ptr);
+  s->bytes += (int64_t)sizeof(T) * n;
+  *p = (T *)ptr;
+  return DGK_OK;
+}
+
+template <typename T>
+int upload(dgk_solver *s, T *dst, const std::vector<T> &src) {
+  if (src.empty()) return DGK_OK;
+  DGK_HIP(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice,
+                         s->stream));
+  return DGK_OK;
+}
+
+hipEvent_t take_event(dgk_solver *s) {
+  if (!s->free_events.empty()) {
+    hipEvent_t e = s->free_events.back();
+    s->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Record an event pair around a launch sequence when timing is on.
+struct Timed {
+  dgk_solver *s;
+  int slot;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(dgk_solver *s_, int slot_, bool on) : s(s_), slot(slot_) {
+    if (!on) return;
+    a = take_event(s);
+    b = take_event(s);
+    if (a && b) (void)hipEventRecord(a, s->stream);
+  }
+  void done() {
+    if (a && b) {
+      (void)hipEventRecord(b, s->stream);
+      s->pending.push_back({slot, a, b});
+      a = b = nullptr;
+    }
+  }
+};
+
+int resolve_timers(dgk_solver *s) {
+  if (s->pending.empty()) return DGK_OK;
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  for (auto &p : s->pending) {
+    float ms = 0.f;
+    DGK_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    s->sec[p.slot] += 1e-3 * (double)ms;
+    s->cnt[p.slot] += 1;
+    s->free_events.push_back(p.a);
+    s->free_events.push_back(p.b);
+  }
+  s->pending.clear();
+  return DGK_OK;
+}
+
+int check_solver(const dgk_solver *s) {
+  if (!s) return set_error(DGK_ERR_ARG, "solver handle is NULL");
+  return DGK_OK;
+}
+
+int enter(const dgk_solver *s) {
+  clear_error();
+  DGK_TRY(check_solver(s));
+  DGK_HIP(hipSetDevice(s->device));
+  return DGK_OK;
+}
+
+int require_fields(const dgk_solver *s) {
+  for (int f = 0; f < 4; ++f)
+    if (!s->set[f])
+      return set_error(DGK_ERR_STATE, "fields q, u, v, w must be set before computing");
+  return DGK_OK;
+}
+
+double *field_ptr(dgk_solver *s, int which) {
+  switch (which) {
+  case DGK_FIELD_Q: return s->q[s->cur];
+  case DGK_FIELD_U: return s->u;
+  case DGK_FIELD_V: return s->v;
+  case DGK_FIELD_W: return s->w;
+  case DGK_FIELD_DQDT: return s->dqdt;
+  default: return nullptr;
+  }
+}
+
+int refresh_f32(dgk_solver *s, int which) {
+  if (s->evaltype != DGK_EVAL_HIP_MP) return DGK_OK;
+  float *dst = which == DGK_FIELD_U ? s->u32 : which == DGK_FIELD_V ? s->v32 : s->w32;
+  return launch_to_f32(field_ptr(s, which), dst, (int64_t)s->Np * s->Ne, s->stream);
+}
+
+StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
+                     double *out, int mode, double ca, double cb, double dt) {
+  StageArgs a;
+  a.q = qin; a.q0 = q0; a.u = s->u; a.v = s->v; a.w = s->w; a.out = out;
+  a.flux_in = nullptr; a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D; a.Lift = s->Lift;
+  a.geo = s->geo; a.geoid = s->geoid; a.Ne = s->Ne;
+  a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode;
+  return a;
+}
+
+int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out,
+                 int mode, double ca, double cb, double dt) {
+  if (s->evaltype == DGK_EVAL_HIP_MP) {
+    StageArgsMP a;
+    a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
+    a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
+    a.geo = s->geo; a.geoid = s->geoid; a.Ne = s->Ne;
+    a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode;
+    return launch_stage_mp(s->Nq, a, s->stream);
+  }
+  return launch_stage_f64(s->Nq, stage_args(s, qin, q0, out, mode, ca, cb, dt), s->stream);
+}
+
+// Separable 1D form of the stored Lift for the mixed-precision path:
+// Lift(:,:,:,f) = I (x) l(-/+) (x) I along the face normal.  Taken from the
+// face-2 (+x) and face-4 (-x) blocks at tangential node (0,0); every block
+// must agree with it to 1e-10 relative.
+int separable_lift(const MeshTables &t, std::vector<float> &ell) {
+  const int Nq = t.Nq, Np = t.Np;
+  std::vector<double> lm(Nq), lp(Nq);
+  for (int a = 0; a < Nq; ++a) {
+    lp[a] = t.Lift[a + 1 * Np];  // face +x: Lift(a,0,0,2) = l+(a)
+    lm[a] = t.Lift[a + 3 * Np];  // face -x: Lift(a,0,0,4) = l-(a)
+  }
+  double scale = 0.0;
+  for (double x : t.Lift) scale = std::fmax(scale, std::fabs(x));
+  double dev = 0.0;
+  for (int f = 0; f < 6; ++f)
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const int axis_idx = (f == 1 || f == 3) ? i : (f == 0 || f == 2) ? j : k;
+          const bool plus = (f == 1 || f == 2 || f == 5);
+          const double ref = plus ? lp[axis_idx] : lm[axis_idx];
+          dev = std::fmax(dev, std::fabs(t.Lift[i + j * Nq + k * Nq * Nq + f * Np] - ref));
+        }
+  if (dev > 1e-10 * scale)
+    return set_error(DGK_ERR_UNSUPPORTED,
+                     "Lift_mat is not separable (deviation " + std::to_string(dev) +
+                         "); HIP_MP needs the tensor-product Lift");
+  ell.resize(2 * Nq);
+  for (int a = 0; a < Nq; ++a) {
+    ell[a] = (float)lm[a];
+    ell[Nq + a] = (float)lp[a];
+  }
+  return DGK_OK;
+}
+
+int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
+  if (!out) return set_error(DGK_ERR_ARG, "output handle pointer is NULL");
+  *out = nullptr;
+  if (evaltype == DGK_EVAL_GENERAL || evaltype == DGK_EVAL_OPT1)
+    return set_error(DGK_ERR_UNSUPPORTED,
+                     "GENERAL and OPT1 name the reference CPU kernels; use HIP or HIP_MP");
+  if (evaltype != DGK_EVAL_HIP && evaltype != DGK_EVAL_HIP_MP)
+    return set_error(DGK_ERR_ARG, "unknown evaluation type id " + std::to_string(evaltype));
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    return set_error(DGK_ERR_DEVICE, "no HIP device available (the path has no CPU fallback)");
+  }
+  std::vector<float> ell;
+  if (evaltype == DGK_EVAL_HIP_MP) DGK_TRY(separable_lift(t, ell));
+
+  auto *s = new dgk_solver();
+  auto fail = [&](int rc) {
+    dgk_destroy(s);
+    return rc;
+  };
+  int rc;
+  s->evaltype = evaltype;
+  if (hipGetDevice(&s->device) != hipSuccess) return fail(set_error(DGK_ERR_DEVICE, "hipGetDevice failed"));
+  s->Nq = t.Nq; s->Np = t.Np; s->Nfp = t.Nfp; s->NfpTot = t.NfpTot;
+  s->Ne = t.Ne; s->NeA = t.NeA; s->NhaloNode = t.NhaloNode;
+  s->nfield = (int64_t)t.Np * t.Ne + t.NhaloNode;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(DGK_ERR_DEVICE, "hipStreamCreate failed"));
+  for (int b = 0; b < 3; ++b)
+    if ((rc = dalloc(s, &s->q[b], s->nfield))) return fail(rc);
+  if ((rc = dalloc(s, &s->u, s->nfield)) || (rc = dalloc(s, &s->v, s->nfield)) ||
+      (rc = dalloc(s, &s->w, s->nfield)))
+    return fail(rc);
+  if ((rc = dalloc(s, &s->nbr, (int64_t)t.nbr.size())) ||
+      (rc = dalloc(s, &s->vmapP, (int64_t)t.vmapP.size())) ||
+      (rc = dalloc(s, &s->halo, (int64_t)t.halo_src.size())) ||
+      (rc = dalloc(s, &s->geo, (int64_t)t.geo.size())) ||
+      (rc = dalloc(s, &s->geoid, (int64_t)t.geoid.size())) ||
+      (rc = dalloc(s, &s->D, (int64_t)t.D.size())) ||
+      (rc = dalloc(s, &s->Lift, (int64_t)t.Lift.size())) ||
+      (rc = dalloc(s, &s->mm, 2 * kMinmaxBlocks + 2)))
+    return fail(rc);
+  if (evaltype == DGK_EVAL_HIP_MP) {
+    const int64_t n = (int64_t)t.Np * t.Ne;
+    if ((rc = dalloc(s, &s->u32, n)) || (rc = dalloc(s, &s->v32, n)) ||
+        (rc = dalloc(s, &s->w32, n)) || (rc = dalloc(s, &s->D32, (int64_t)t.D.size())) ||
+        (rc = dalloc(s, &s->ell32, (int64_t)ell.size())))
+      return fail(rc);
+    std::vector<float> D32(t.D.begin(), t.D.end());
+    if ((rc = upload(s, s->D32, D32)) || (rc = upload(s, s->ell32, ell))) return fail(rc);
+  }
+  if ((rc = upload(s, s->nbr, t.nbr)) || (rc = upload(s, s->vmapP, t.vmapP)) ||
+      (rc = upload(s, s->halo, t.halo_src)) || (rc = upload(s, s->geo, t.geo)) ||
+      (rc = upload(s, s->geoid, t.geoid)) || (rc = upload(s, s->D, t.D)) ||
+      (rc = upload(s, s->Lift, t.Lift)))
+    return fail(rc);
+  for (int b = 0; b < 3; ++b)
+    if (hipMemsetAsync(s->q[b], 0, sizeof(double) * s->nfield, s->stream) != hipSuccess)
+      return fail(set_error(DGK_ERR_DEVICE, "hipMemsetAsync failed"));
+  if (hipStreamSynchronize(s->stream) != hipSuccess)
+    return fail(set_error(DGK_ERR_DEVICE, "stream synchronize failed after upload"));
+  *out = s;
+  return DGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dgk_last_error(void) { return g_last_error.c_str(); }
+
+int dgk_version(void) { return 100; }
+
+int dgk_device_count(int *count) {
+  clear_error();
+  if (!count) return set_error(DGK_ERR_ARG, "count is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  *count = n;
+  return DGK_OK;
+}
+
+int dgk_evaltype_id(const char *name, int *id) {
+  clear_error();
+  if (!name || !id) return set_error(DGK_ERR_ARG, "null argument");
+  // trim trailing blanks, as Fortran's trim() does (mod_dg_optr_kernel.f90:50)
+  std::string s(name);
+  while (!s.empty() && (s.back() == ' ' || s.back() == '\0')) s.pop_back();
+  if (s == "GENERAL") *id = DGK_EVAL_GENERAL;
+  else if (s == "OPT1") *id = DGK_EVAL_OPT1;
+  else if (s == "HIP") *id = DGK_EVAL_HIP;
+  else if (s == "HIP_MP") *id = DGK_EVAL_HIP_MP;
+  else return set_error(DGK_ERR_ARG, "Unsupported dgoptr_evaltype: " + s);
+  return DGK_OK;
+}
+
+int dgk_create(int evaltype, int Nq, int64_t Ne, int64_t NeA, int64_t NhaloNode,
+               const double *D1D, const double *Lift_mat, const int *VMapM,
+               const int *VMapP, const int *halo_src_map, const double *normal_fn,
+               const double *Escale, const double *Fscale, dgk_solver **out) {
+  clear_error();
+  if (!out) return set_error(DGK_ERR_ARG, "output handle pointer is NULL");
+  *out = nullptr;
+  MeshTables t;
+  DGK_TRY(build_tables_from_reference(Nq, Ne, NeA, NhaloNode, D1D, Lift_mat, VMapM,
+                                      VMapP, halo_src_map, normal_fn, Escale, Fscale, t));
+  return create_from_tables(evaltype, t, out);
+}
+
+int dgk_create_structured(int evaltype, int NeX, int NeY, int NeZ, int p,
+                          const char *operator_dir, double dom_x, double dom_y,
+                          double dom_z, dgk_solver **out) {
+  clear_error();
+  if (!out) return set_error(DGK_ERR_ARG, "output handle pointer is NULL");
+  *out = nullptr;
+  std::vector<double> x, D, L;
+  DGK_TRY(load_operators(operator_dir, p, x, D, L));
+  MeshTables t;
+  DGK_TRY(build_tables_structured(NeX, NeY, NeZ, p, D.data(), L.data(), dom_x, dom_y,
+                                  dom_z, t));
+  return create_from_tables(evaltype, t, out);
+}
+
+int dgk_destroy(dgk_solver *s) {
+  clear_error();
+  if (!s) return DGK_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto &p : s->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+  for (void *p : s->allocs) (void)hipFree(p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return DGK_OK;
+}
+
+int dgk_solver_info(const dgk_solver *s, int *Nq, int64_t *Ne, int64_t *NeA,
+                    int64_t *NhaloNode) {
+  clear_error();
+  DGK_TRY(check_solver(s));
+  if (Nq) *Nq = s->Nq;
+  if (Ne) *Ne = s->Ne;
+  if (NeA) *NeA = s->NeA;
+  if (NhaloNode) *NhaloNode = s->NhaloNode;
+  return DGK_OK;
+}
+
+int dgk_device_bytes(const dgk_solver *s, int64_t *bytes) {
+  clear_error();
+  DGK_TRY(check_solver(s));
+  if (!bytes) return set_error(DGK_ERR_ARG, "bytes is NULL");
+  *bytes = s->bytes;
+  return DGK_OK;
+}
+
+int dgk_set_field(dgk_solver *s, int which, const double *f, int64_t n) {
+  DGK_TRY(enter(s));
+  if (!f) return set_error(DGK_ERR_ARG, "field array is NULL");
+  if (which < DGK_FIELD_Q || which > DGK_FIELD_W)
+    return set_error(DGK_ERR_ARG, "field id must be Q, U, V or W");
+  const int64_t owned = (int64_t)s->Np * s->Ne;
+  if (n != owned && n != (int64_t)s->Np * s->NeA)
+    return set_error(DGK_ERR_ARG, "field length must be Np*Ne or Np*NeA");
+  double *dst = field_ptr(s, which);
+  DGK_HIP(hipMemcpyAsync(dst, f, sizeof(double) * owned, hipMemcpyHostToDevice, s->stream));
+  if (which != DGK_FIELD_Q) {
+    DGK_TRY(launch_halo_gather(dst, s->halo, owned, s->NhaloNode, s->stream));
+    DGK_TRY(refresh_f32(s, which));
+  } else {
+    s->q0_valid = false;
+  }
+  s->flux_valid = s->dqdt_valid = false;
+  s->set[which] = true;
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_get_field(dgk_solver *s, int which, double *f, int64_t n) {
+  DGK_TRY(enter(s));
+  if (!f) return set_error(DGK_ERR_ARG, "field array is NULL");
+  if (which < DGK_FIELD_Q || which > DGK_FIELD_DQDT)
+    return set_error(DGK_ERR_ARG, "unknown field id");
+  const int64_t owned = (int64_t)s->Np * s->Ne;
+  const bool with_halo = n == (int64_t)s->Np * s->NeA;
+  if (n != owned && !with_halo) return set_error(DGK_ERR_ARG, "field length must be Np*Ne or Np*NeA");
+  if (which == DGK_FIELD_DQDT && !s->dqdt) return set_error(DGK_ERR_STATE, "dqdt has not been computed");
+  const double *src = field_ptr(s, which);
+  DGK_HIP(hipMemcpyAsync(f, src, sizeof(double) * owned, hipMemcpyDeviceToHost, s->stream));
+  if (with_halo) {
+    const int64_t tail = (int64_t)s->Np * s->NeA - owned;
+    if (which == DGK_FIELD_DQDT) {
+      std::memset(f + owned, 0, sizeof(double) * tail);
+    } else {
+      DGK_HIP(hipMemcpyAsync(f + owned, src + owned, sizeof(double) * s->NhaloNode,
+                             hipMemcpyDeviceToHost, s->stream));
+      if (tail > s->NhaloNode)
+        std::memset(f + owned + s->NhaloNode, 0, sizeof(double) * (tail - s->NhaloNode));
+    }
+  }
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_update_halo(dgk_solver *s, int which) {
+  DGK_TRY(enter(s));
+  if (which < DGK_FIELD_Q || which > DGK_FIELD_W)
+    return set_error(DGK_ERR_ARG, "field id must be Q, U, V or W");
+  return launch_halo_gather(field_ptr(s, which), s->halo, (int64_t)s->Np * s->Ne,
+                            s->NhaloNode, s->stream);
+}
+
+int dgk_copy_q0(dgk_solver *s) {
+  DGK_TRY(enter(s));
+  DGK_TRY(require_fields(s));
+  const int64_t owned = (int64_t)s->Np * s->Ne;
+  if (!s->q0) DGK_TRY(dalloc(s, &s->q0, owned));
+  DGK_HIP(hipMemcpyAsync(s->q0, s->q[s->cur], sizeof(double) * owned,
+                         hipMemcpyDeviceToDevice, s->stream));
+  s->q0_valid = true;
+  return DGK_OK;
+}
+
+int dgk_cal_elembnd_flux(dgk_solver *s, double *flux_host) {
+  DGK_TRY(enter(s));
+  DGK_TRY(require_fields(s));
+  const int64_t n = (int64_t)s->NfpTot * s->Ne;
+  if (!s->flux) DGK_TRY(dalloc(s, &s->flux, n));
+  Timed tm(s, T_FLUX, s->kernel_timing);
+  DGK_TRY(launch_elembnd_flux_f64(s->Nq, s->q[s->cur], s->u, s->v, s->w, s->nbr, s->vmapP,
+                                  s->geo, s->geoid, s->Ne, s->flux, s->stream));
+  tm.done();
+  s->flux_valid = true;
+  if (flux_host) {
+    DGK_HIP(hipMemcpyAsync(flux_host, s->flux, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           s->stream));
+    DGK_HIP(hipStreamSynchronize(s->stream));
+  }
+  return DGK_OK;
+}
+
+int dgk_cal_dqdt(dgk_solver *s, const double *flux_host) {
+  DGK_TRY(enter(s));
+  DGK_TRY(require_fields(s));
+  if (s->evaltype != DGK_EVAL_HIP)
+    return set_error(DGK_ERR_UNSUPPORTED, "the unfused cal_dqdt is provided by the FP64 path only");
+  const int64_t n = (int64_t)s->NfpTot * s->Ne;
+  if (!s->flux) DGK_TRY(dalloc(s, &s->flux, n));
+  if (flux_host) {
+    DGK_HIP(hipMemcpyAsync(s->flux, flux_host, sizeof(double) * n, hipMemcpyHostToDevice,
+                           s->stream));
+    s->flux_valid = true;
+  }
+  if (!s->flux_valid) return set_error(DGK_ERR_STATE, "no element boundary flux computed");
+  if (!s->dqdt) DGK_TRY(dalloc(s, &s->dqdt, (int64_t)s->Np * s->Ne));
+  StageArgs a = stage_args(s, s->q[s->cur], nullptr, s->dqdt, 0, 0.0, 1.0, 0.0);
+  a.flux_in = s->flux;
+  Timed tm(s, T_DQDT, s->kernel_timing);
+  DGK_TRY(launch_stage_f64(s->Nq, a, s->stream));
+  tm.done();
+  s->dqdt_valid = true;
+  if (flux_host) DGK_HIP(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_cal_tend(dgk_solver *s, double *dqdt_host) {
+  DGK_TRY(enter(s));
+  DGK_TRY(require_fields(s));
+  const int64_t n = (int64_t)s->Np * s->Ne;
+  if (!s->dqdt) DGK_TRY(dalloc(s, &s->dqdt, n));
+  Timed tm(s, T_CALTEND, s->kernel_timing);
+  DGK_TRY(launch_stage(s, s->q[s->cur], nullptr, s->dqdt, 0, 0.0, 1.0, 0.0));
+  tm.done();
+  s->dqdt_valid = true;
+  if (dqdt_host) {
+    DGK_HIP(hipMemcpyAsync(dqdt_host, s->dqdt, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           s->stream));
+    DGK_HIP(hipStreamSynchronize(s->stream));
+  }
+  return DGK_OK;
+}
+
+int dgk_rk_update(dgk_solver *s, int stage, double dt) {
+  DGK_TRY(enter(s));
+  if (stage < 1 || stage > 3) return set_error(DGK_ERR_ARG, "stage must be 1, 2 or 3");
+  if (!s->q0_valid) return set_error(DGK_ERR_STATE, "q0 not set (dgk_copy_q0)");
+  if (!s->dqdt_valid) return set_error(DGK_ERR_STATE, "dqdt not computed");
+  DGK_TRY(launch_rk_update(s->q[s->cur], s->q0, s->dqdt, kRkA[stage - 1], kRkB[stage - 1], dt,
+                           (int64_t)s->Np * s->Ne, s->stream));
+  s->flux_valid = s->dqdt_valid = false;
+  return DGK_OK;
+}
+
+int dgk_step(dgk_solver *s, double dt, int nstep) {
+  DGK_TRY(enter(s));
+  DGK_TRY(require_fields(s));
+  if (nstep < 0) return set_error(DGK_ERR_ARG, "nstep must be >= 0");
+  if (nstep == 0) return DGK_OK;
+  // Buffer rotation: A holds q at the step start and serves as q0; stage 1
+  // A->B, stage 2 (B, q0=A)->C, stage 3 (C, q0=A)->A.  Stage 3 overwrites
+  // A node-for-node after reading q0 there, and no element reads another
+  // element's q0, so no copy of q0 is needed.
+  double *A = s->q[s->cur], *B = s->q[(s->cur + 1) % 3], *C = s->q[(s->cur + 2) % 3];
+  Timed main_tm(s, T_MAIN, true);
+  for (int it = 0; it < nstep; ++it) {
+    const double *in[3] = {A, B, C};
+    double *out[3] = {B, C, A};
+    for (int st = 0; st < 3; ++st) {
+      Timed tm(s, T_STAGE, s->kernel_timing);
+      DGK_TRY(launch_stage(s, in[st], A, out[st], 1, kRkA[st], kRkB[st], dt));
+      tm.done();
+    }
+  }
+  main_tm.done();
+  s->q0_valid = s->flux_valid = s->dqdt_valid = false;
+  return DGK_OK;
+}
+
+int dgk_minmax(dgk_solver *s, double *qmin, double *qmax) {
+  DGK_TRY(enter(s));
+  if (!s->set[DGK_FIELD_Q]) return set_error(DGK_ERR_STATE, "q not set");
+  const int64_t n = (int64_t)s->Np * s->Ne;
+  DGK_TRY(launch_minmax(s->q[s->cur], n, s->mm, kMinmaxBlocks, s->mm + 2 * kMinmaxBlocks,
+                        s->stream));
+  double r[2];
+  DGK_HIP(hipMemcpyAsync(r, s->mm + 2 * kMinmaxBlocks, sizeof(r), hipMemcpyDeviceToHost,
+                         s->stream));
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  if (qmin) *qmin = r[0];
+  if (qmax) *qmax = r[1];
+  return DGK_OK;
+}
+
+int dgk_synchronize(dgk_solver *s) {
+  DGK_TRY(enter(s));
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  return resolve_timers(s);
+}
+
+int dgk_set_kernel_timing(dgk_solver *s, int on) {
+  DGK_TRY(enter(s));
+  s->kernel_timing = on != 0;
+  return DGK_OK;
+}
+
+int dgk_timer_report(dgk_solver *s, double *seconds, int64_t *counts) {
+  DGK_TRY(enter(s));
+  DGK_TRY(resolve_timers(s));
+  for (int i = 0; i < T_N; ++i) {
+    if (seconds) seconds[i] = s->sec[i];
+    if (counts) counts[i] = s->cnt[i];
+  }
+  return DGK_OK;
+}
+
+int dgk_timer_reset(dgk_solver *s) {
+  DGK_TRY(enter(s));
+  DGK_TRY(resolve_timers(s));
+  for (int i = 0; i < T_N; ++i) {
+    s->sec[i] = 0.0;
+    s->cnt[i] = 0;
+  }
+  return DGK_OK;
+}
+
+}  // extern "C"

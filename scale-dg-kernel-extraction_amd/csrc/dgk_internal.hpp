@@ -1,0 +1,152 @@
+// Internal declarations shared by the host side and the HIP kernels of libdgk.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dgk.h"
+
+namespace dgk {
+
+// ---------------------------------------------------------------------------
+// Error reporting: every C entry point returns a status and leaves a message
+// in a thread-local buffer (dgk_last_error).
+// ---------------------------------------------------------------------------
+int set_error(int code, const std::string &msg);
+void clear_error();
+
+#define DGK_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::dgk::set_error(DGK_ERR_DEVICE, std::string(#call) + ": " +      \
+                                                  hipGetErrorString(e_));      \
+  } while (0)
+
+#define DGK_TRY(call)                                                          \
+  do {                                                                         \
+    int rc_ = (call);                                                          \
+    if (rc_ != DGK_OK) return rc_;                                             \
+  } while (0)
+
+constexpr int kMinNq = 2;
+constexpr int kMaxNq = 16;
+
+// Per-element geometry record: Escale[3] (2/dx_d, constant per element,
+// mod_mesh.f90:147-149), Fscale[6] and the outward normal of each face
+// (set_face_geometry, mod_mesh.f90:356-397).  Identical records are shared.
+constexpr int kGeoStride = 32;  // doubles per record (27 used)
+constexpr int kGeoEscale = 0;
+constexpr int kGeoFscale = 3;
+constexpr int kGeoNormal = 9;  // + 3*f + d
+
+// Node index (0-based) inside an element of face node (a,b) of face f on the
+// element's own side, in the reference's Fmask order (mod_mesh.f90:278-324):
+// faces 0..5 = -y, +x, +y, -x, -z, +z.
+__host__ __device__ inline int own_face_node(int f, int a, int b, int Nq) {
+  const int Nq2 = Nq * Nq;
+  switch (f) {
+  case 0: return a + b * Nq2;                       // (a, 0, b)
+  case 1: return (Nq - 1) + a * Nq + b * Nq2;       // (Nq-1, a, b)
+  case 2: return a + (Nq - 1) * Nq + b * Nq2;       // (a, Nq-1, b)
+  case 3: return a * Nq + b * Nq2;                  // (0, a, b)
+  case 4: return a + b * Nq;                        // (a, b, 0)
+  default: return a + b * Nq + (Nq - 1) * Nq2;      // (a, b, Nq-1)
+  }
+}
+// Opposite face (mod_mesh.f90:420-480): 0<->2, 1<->3, 4<->5.
+__host__ __device__ inline int opp_face(int f) {
+  return f < 4 ? (f + 2) % 4 : (f == 4 ? 5 : 4);
+}
+// Node of the neighbour that face node (a,b) of face f meets: the same (a,b)
+// on the neighbour's opposite face (interior VMapP, mod_mesh.f90:163).
+__host__ __device__ inline int nbr_face_node(int f, int a, int b, int Nq) {
+  return own_face_node(opp_face(f), a, b, Nq);
+}
+
+// Compact device description of the mesh, built from the reference arrays.
+struct MeshTables {
+  int Nq = 0, Np = 0, Nfp = 0, NfpTot = 0;
+  int64_t Ne = 0, NeA = 0, NhaloNode = 0;
+  // Connectivity: either a neighbour table nbr[e*6+f] (0-based element) when
+  // every face meets the opposite face of one element node-for-node, or the
+  // resolved 0-based P index of every face node (halo folded in).
+  std::vector<int32_t> nbr;
+  std::vector<int32_t> vmapP;
+  std::vector<int32_t> halo_src;  // 0-based, NhaloNode
+  std::vector<double> geo;        // ngeo * kGeoStride
+  std::vector<uint32_t> geoid;    // Ne, empty when ngeo == 1
+  std::vector<double> D;          // Nq*Nq column-major
+  std::vector<double> Lift;       // Nq^3*6 column-major
+};
+
+int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
+                                int64_t NhaloNode, const double *D1D,
+                                const double *Lift_mat, const int *VMapM,
+                                const int *VMapP, const int *halo_src_map,
+                                const double *normal_fn, const double *Escale,
+                                const double *Fscale, MeshTables &t);
+int build_tables_structured(int NeX, int NeY, int NeZ, int p,
+                            const double *D1D, const double *Lift_mat,
+                            double dom_x, double dom_y, double dom_z,
+                            MeshTables &t);
+int load_operators(const char *dir, int p, std::vector<double> &x1D,
+                   std::vector<double> &D, std::vector<double> &Lift);
+
+// ---------------------------------------------------------------------------
+// Kernel parameter blocks and launchers (kernels_f64.hip, kernels_mp.hip).
+// ---------------------------------------------------------------------------
+struct StageArgs {
+  const double *q;        // stage input, owned part (Np*Ne)
+  const double *q0;       // RK base (may alias out); unused when ca == 0
+  const double *u, *v, *w;
+  double *out;            // dqdt (mode 0) or RK result (mode 1)
+  const double *flux_in;  // face flux (NfpTot,Ne) to use instead of computing it
+  const int32_t *nbr;     // compact connectivity or nullptr
+  const int32_t *vmapP;   // node connectivity or nullptr
+  const double *D;        // Nq*Nq
+  const double *Lift;     // Nq^3*6
+  const double *geo;
+  const uint32_t *geoid;  // nullptr when one record
+  int64_t Ne;
+  double ca, cb, dt;      // RK stage: out = ca*q0 + cb*(q + dt*tend)
+  int mode;               // 0 = tendency, 1 = RK stage
+};
+
+// Mixed-precision operands (FP32 copies of the constant inputs).
+struct StageArgsMP {
+  const double *q;
+  const double *q0;
+  const float *u, *v, *w;
+  double *out;
+  const int32_t *nbr;
+  const int32_t *vmapP;
+  const float *D;         // Nq*Nq, FP32
+  const float *ell;       // 2*Nq: l-(x) then l+(x), the separable 1D Lift
+  const double *geo;
+  const uint32_t *geoid;
+  int64_t Ne;
+  double ca, cb, dt;
+  int mode;
+};
+
+int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);
+int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
+int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,
+                            const double *v, const double *w,
+                            const int32_t *nbr, const int32_t *vmapP,
+                            const double *geo, const uint32_t *geoid,
+                            int64_t Ne, double *flux, hipStream_t s);
+int launch_halo_gather(double *f, const int32_t *src, int64_t base,
+                       int64_t n, hipStream_t s);
+int launch_rk_update(double *q, const double *q0, const double *dqdt,
+                     double a, double b, double dt, int64_t n, hipStream_t s);
+int launch_minmax(const double *q, int64_t n, double *partial, int nblk,
+                  double *result, hipStream_t s);
+int launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s);
+constexpr int kMinmaxBlocks = 1024;
+
+}  // namespace dgk

@@ -1,0 +1,367 @@
+// Host-side mesh and operator handling for libdgk.
+//
+//  * load_operators      : operator_data/p{p}.dat reader (element_init,
+//                          mod_mesh.f90:253-263).
+//  * dgk_mesh_build      : the reference's per-node mesh arrays for a periodic
+//                          box (mesh_setup, mod_mesh.f90:74-193), for callers
+//                          that do not have the reference host at hand.
+//  * build_tables_*      : compression of those arrays into what the kernels
+//                          read: a 6-entry neighbour table per element (or a
+//                          resolved per-face-node index), the halo folded into
+//                          it, and deduplicated per-element geometry records.
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+
+#include "dgk_internal.hpp"
+
+namespace dgk {
+
+namespace {
+
+// One list-directed record: the first n numbers of the next line(s).
+bool read_record(std::ifstream &in, double *dst, size_t n) {
+  size_t got = 0;
+  std::string line;
+  while (got < n) {
+    if (!std::getline(in, line)) return false;
+    const char *s = line.c_str();
+    char *end = nullptr;
+    while (got < n) {
+      double v = std::strtod(s, &end);
+      if (end == s) break;
+      dst[got++] = v;
+      s = end;
+    }
+  }
+  return true;
+}
+
+// Reference face order -y, +x, +y, -x, -z, +z: normal axis and sign.
+constexpr int kFaceAxis[6] = {1, 0, 1, 0, 2, 2};
+constexpr double kFaceSign[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+
+// Periodic neighbour of element (i,j,k) across face f (0-based indices,
+// eval_neighbor_element, mod_mesh.f90:407-483).  Returns the neighbour's
+// element index and whether the face lies on the domain boundary.
+int64_t neighbour(int i, int j, int k, int f, int NeX, int NeY, int NeZ,
+                  bool &boundary) {
+  boundary = false;
+  switch (f) {
+  case 0: if (j > 0) --j; else { j = NeY - 1; boundary = true; } break;
+  case 1: if (i < NeX - 1) ++i; else { i = 0; boundary = true; } break;
+  case 2: if (j < NeY - 1) ++j; else { j = 0; boundary = true; } break;
+  case 3: if (i > 0) --i; else { i = NeX - 1; boundary = true; } break;
+  case 4: if (k > 0) --k; else { k = NeZ - 1; boundary = true; } break;
+  default: if (k < NeZ - 1) ++k; else { k = 0; boundary = true; } break;
+  }
+  return i + (int64_t)j * NeX + (int64_t)k * NeX * NeY;
+}
+
+void fill_box_geo(double dx, double dy, double dz, double *rec) {
+  const double h[3] = {dx, dy, dz};
+  std::memset(rec, 0, sizeof(double) * kGeoStride);
+  for (int d = 0; d < 3; ++d) rec[kGeoEscale + d] = 2.0 / h[d];
+  for (int f = 0; f < 6; ++f) {
+    rec[kGeoFscale + f] = 2.0 / h[kFaceAxis[f]];
+    rec[kGeoNormal + 3 * f + kFaceAxis[f]] = kFaceSign[f];
+  }
+}
+
+struct RecHash {
+  size_t operator()(const std::array<uint64_t, 27> &a) const {
+    uint64_t h = 1469598103934665603ULL;
+    for (uint64_t v : a) { h ^= v; h *= 1099511628211ULL; }
+    return (size_t)h;
+  }
+};
+
+}  // namespace
+
+int load_operators(const char *dir, int p, std::vector<double> &x1D,
+                   std::vector<double> &D, std::vector<double> &Lift) {
+  if (!dir) return set_error(DGK_ERR_ARG, "operator directory is NULL");
+  if (p < kMinNq - 1 || p > kMaxNq - 1)
+    return set_error(DGK_ERR_UNSUPPORTED,
+                     "polynomial order " + std::to_string(p) + " outside 1..15");
+  const int Nq = p + 1;
+  const std::string path = std::string(dir) + "/p" + std::to_string(p) + ".dat";
+  std::ifstream in(path);
+  if (!in) return set_error(DGK_ERR_IO, "cannot open operator file " + path);
+  x1D.assign(Nq, 0.0);
+  D.assign((size_t)Nq * Nq, 0.0);
+  Lift.assign((size_t)Nq * Nq * Nq * 6, 0.0);
+  if (!read_record(in, x1D.data(), x1D.size()) ||
+      !read_record(in, D.data(), D.size()) ||
+      !read_record(in, Lift.data(), Lift.size()))
+    return set_error(DGK_ERR_IO, "operator file " + path + " is truncated");
+  return DGK_OK;
+}
+
+int build_tables_structured(int NeX, int NeY, int NeZ, int p,
+                            const double *D1D, const double *Lift_mat,
+                            double dom_x, double dom_y, double dom_z,
+                            MeshTables &t) {
+  if (NeX < 1 || NeY < 1 || NeZ < 1)
+    return set_error(DGK_ERR_ARG, "element counts must be >= 1");
+  if (p < kMinNq - 1 || p > kMaxNq - 1)
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+  if (!(dom_x > 0 && dom_y > 0 && dom_z > 0))
+    return set_error(DGK_ERR_ARG, "domain sizes must be positive");
+  const int Nq = p + 1;
+  t.Nq = Nq; t.Np = Nq * Nq * Nq; t.Nfp = Nq * Nq; t.NfpTot = 6 * t.Nfp;
+  t.Ne = (int64_t)NeX * NeY * NeZ;
+  const int64_t Nhalo = 2 * ((int64_t)NeY * NeZ + (int64_t)NeX * NeZ + (int64_t)NeX * NeY);
+  t.NeA = t.Ne + Nhalo;
+  t.NhaloNode = (int64_t)t.Nfp * Nhalo;
+  if ((int64_t)t.Np * t.NeA >= (int64_t)INT32_MAX)
+    return set_error(DGK_ERR_UNSUPPORTED, "mesh exceeds 2^31 nodes (int32 maps)");
+  t.D.assign(D1D, D1D + (size_t)Nq * Nq);
+  t.Lift.assign(Lift_mat, Lift_mat + (size_t)t.Np * 6);
+  t.nbr.resize((size_t)t.Ne * 6);
+  t.vmapP.clear();
+  t.halo_src.resize(t.NhaloNode);
+  int64_t ibuf = 0;
+  for (int k = 0; k < NeZ; ++k)
+    for (int j = 0; j < NeY; ++j)
+      for (int i = 0; i < NeX; ++i) {
+        const int64_t e = i + (int64_t)j * NeX + (int64_t)k * NeX * NeY;
+        for (int f = 0; f < 6; ++f) {
+          bool bnd;
+          const int64_t nb = neighbour(i, j, k, f, NeX, NeY, NeZ, bnd);
+          t.nbr[e * 6 + f] = (int32_t)nb;
+          if (bnd)  // halo slots in serialized (k,j,i,f,fp) order
+            for (int fp = 0; fp < t.Nfp; ++fp)
+              t.halo_src[ibuf++] =
+                  (int32_t)(nb * t.Np + nbr_face_node(f, fp % Nq, fp / Nq, Nq));
+        }
+      }
+  t.geo.assign(kGeoStride, 0.0);
+  fill_box_geo(dom_x / (double)NeX, dom_y / (double)NeY, dom_z / (double)NeZ,
+               t.geo.data());
+  t.geoid.clear();
+  return DGK_OK;
+}
+
+int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
+                                int64_t NhaloNode, const double *D1D,
+                                const double *Lift_mat, const int *VMapM,
+                                const int *VMapP, const int *halo_src_map,
+                                const double *normal_fn, const double *Escale,
+                                const double *Fscale, MeshTables &t) {
+  if (!D1D || !Lift_mat || !VMapM || !VMapP || !normal_fn || !Escale || !Fscale ||
+      (NhaloNode > 0 && !halo_src_map))
+    return set_error(DGK_ERR_ARG, "null array argument");
+  if (Nq < kMinNq || Nq > kMaxNq)
+    return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16 (p 1..15)");
+  if (Ne < 1 || NeA < Ne || NhaloNode < 0)
+    return set_error(DGK_ERR_ARG, "inconsistent Ne / NeA / NhaloNode");
+  const int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NfpTot = 6 * Nfp;
+  if ((int64_t)Np * Ne + NhaloNode > (int64_t)Np * NeA)
+    return set_error(DGK_ERR_ARG, "halo does not fit in Np*NeA");
+  if ((int64_t)Np * NeA >= (int64_t)INT32_MAX)
+    return set_error(DGK_ERR_UNSUPPORTED, "mesh exceeds 2^31 nodes (int32 maps)");
+  t.Nq = Nq; t.Np = Np; t.Nfp = Nfp; t.NfpTot = NfpTot;
+  t.Ne = Ne; t.NeA = NeA; t.NhaloNode = NhaloNode;
+  t.D.assign(D1D, D1D + (size_t)Nq * Nq);
+  t.Lift.assign(Lift_mat, Lift_mat + (size_t)Np * 6);
+
+  const int64_t owned = (int64_t)Np * Ne;
+  t.halo_src.resize(NhaloNode);
+  for (int64_t ib = 0; ib < NhaloNode; ++ib) {
+    const int64_t s = (int64_t)halo_src_map[ib] - 1;
+    if (s < 0 || s >= owned)
+      return set_error(DGK_ERR_UNSUPPORTED,
+                       "halo_src_map entry " + std::to_string(ib + 1) +
+                           " does not address an owned node");
+    t.halo_src[ib] = (int32_t)s;
+  }
+
+  // VMapM must address the element's own face nodes in Fmask order
+  // (mod_mesh.f90:159); VMapP is resolved through the halo (update_halo
+  // followed by the read at VMapP equals a direct read at the source).
+  std::vector<int32_t> resolved((size_t)NfpTot * Ne);
+  bool compact = true;
+  std::vector<int32_t> nbr((size_t)Ne * 6);
+  for (int64_t e = 0; e < Ne; ++e) {
+    for (int f = 0; f < 6; ++f) {
+      int64_t nb = -1;
+      for (int fp = 0; fp < Nfp; ++fp) {
+        const int a = fp % Nq, b = fp / Nq;
+        const int64_t r = (int64_t)f * Nfp + fp + e * NfpTot;
+        const int64_t m = (int64_t)VMapM[r] - 1;
+        if (m != e * Np + own_face_node(f, a, b, Nq))
+          return set_error(DGK_ERR_UNSUPPORTED,
+                           "VMapM does not address the element's own Fmask node "
+                           "at face node " + std::to_string(r + 1));
+        int64_t pidx = (int64_t)VMapP[r] - 1;
+        if (pidx < 0 || pidx >= owned + NhaloNode)
+          return set_error(DGK_ERR_ARG, "VMapP entry " + std::to_string(r + 1) +
+                                            " out of range");
+        if (pidx >= owned) pidx = t.halo_src[pidx - owned];
+        resolved[r] = (int32_t)pidx;
+        if (compact) {
+          const int64_t off = pidx - nbr_face_node(f, a, b, Nq);
+          if (off < 0 || off % Np != 0) compact = false;
+          else if (fp == 0) nb = off / Np;
+          else if (off / Np != nb) compact = false;
+        }
+      }
+      if (compact) nbr[e * 6 + f] = (int32_t)nb;
+    }
+  }
+  if (compact) { t.nbr.swap(nbr); t.vmapP.clear(); }
+  else { t.vmapP.swap(resolved); t.nbr.clear(); }
+
+  // Geometry: Escale constant per element and direction, Fscale and the
+  // normal constant per face (affine hexahedra, mod_mesh.f90:147-149,364-394).
+  std::unordered_map<std::array<uint64_t, 27>, uint32_t, RecHash> ids;
+  std::vector<double> geo;
+  std::vector<uint32_t> geoid(Ne);
+  std::array<double, 27> rec;
+  for (int64_t e = 0; e < Ne; ++e) {
+    for (int d = 0; d < 3; ++d) {
+      const double *es = Escale + e * Np + (int64_t)d * Np * Ne;
+      for (int n = 1; n < Np; ++n)
+        if (es[n] != es[0])
+          return set_error(DGK_ERR_UNSUPPORTED,
+                           "Escale varies inside element " + std::to_string(e + 1) +
+                               " (only affine hexahedra are supported)");
+      rec[kGeoEscale + d] = es[0];
+    }
+    for (int f = 0; f < 6; ++f) {
+      const int64_t r0 = (int64_t)f * Nfp + e * NfpTot;
+      for (int fp = 1; fp < Nfp; ++fp) {
+        bool same = Fscale[r0 + fp] == Fscale[r0];
+        for (int d = 0; d < 3; ++d)
+          same = same && normal_fn[r0 + fp + (int64_t)d * NfpTot * Ne] ==
+                             normal_fn[r0 + (int64_t)d * NfpTot * Ne];
+        if (!same)
+          return set_error(DGK_ERR_UNSUPPORTED,
+                           "Fscale/normal_fn vary across face " + std::to_string(f + 1) +
+                               " of element " + std::to_string(e + 1));
+      }
+      rec[kGeoFscale + f] = Fscale[r0];
+      for (int d = 0; d < 3; ++d)
+        rec[kGeoNormal + 3 * f + d] = normal_fn[r0 + (int64_t)d * NfpTot * Ne];
+    }
+    std::array<uint64_t, 27> key;
+    std::memcpy(key.data(), rec.data(), sizeof(key));
+    auto it = ids.find(key);
+    uint32_t id;
+    if (it == ids.end()) {
+      id = (uint32_t)ids.size();
+      ids.emplace(key, id);
+      geo.resize((size_t)(id + 1) * kGeoStride, 0.0);
+      std::memcpy(&geo[(size_t)id * kGeoStride], rec.data(), sizeof(rec));
+    } else {
+      id = it->second;
+    }
+    geoid[e] = id;
+  }
+  t.geo.swap(geo);
+  if (ids.size() == 1) t.geoid.clear();
+  else t.geoid.swap(geoid);
+  return DGK_OK;
+}
+
+}  // namespace dgk
+
+using namespace dgk;
+
+extern "C" int dgk_load_operators(const char *dir, int p, double *x1D,
+                                  double *D1D, double *Lift_mat) {
+  clear_error();
+  if (!x1D || !D1D || !Lift_mat) return set_error(DGK_ERR_ARG, "null output array");
+  std::vector<double> x, D, L;
+  DGK_TRY(load_operators(dir, p, x, D, L));
+  std::memcpy(x1D, x.data(), x.size() * sizeof(double));
+  std::memcpy(D1D, D.data(), D.size() * sizeof(double));
+  std::memcpy(Lift_mat, L.data(), L.size() * sizeof(double));
+  return DGK_OK;
+}
+
+extern "C" int dgk_element_sizes(int p, int *Nq, int *Np, int *Nfp, int *NfpTot) {
+  clear_error();
+  if (p < kMinNq - 1 || p > kMaxNq - 1)
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+  const int q = p + 1;
+  if (Nq) *Nq = q;
+  if (Np) *Np = q * q * q;
+  if (Nfp) *Nfp = q * q;
+  if (NfpTot) *NfpTot = 6 * q * q;
+  return DGK_OK;
+}
+
+extern "C" int dgk_mesh_sizes(int NeX, int NeY, int NeZ, int p, int64_t *Ne,
+                              int64_t *NeA, int64_t *NhaloNode) {
+  clear_error();
+  if (NeX < 1 || NeY < 1 || NeZ < 1) return set_error(DGK_ERR_ARG, "element counts must be >= 1");
+  if (p < kMinNq - 1 || p > kMaxNq - 1)
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+  const int64_t ne = (int64_t)NeX * NeY * NeZ;
+  const int64_t nh = 2 * ((int64_t)NeY * NeZ + (int64_t)NeX * NeZ + (int64_t)NeX * NeY);
+  if (Ne) *Ne = ne;
+  if (NeA) *NeA = ne + nh;
+  if (NhaloNode) *NhaloNode = (int64_t)(p + 1) * (p + 1) * nh;
+  return DGK_OK;
+}
+
+extern "C" int dgk_mesh_build(int NeX, int NeY, int NeZ, int p,
+                              const double *x1D, double dom_x, double dom_y,
+                              double dom_z, int *VMapM, int *VMapP,
+                              int *halo_src_map, double *normal_fn,
+                              double *Escale, double *Fscale, double *pos_en) {
+  clear_error();
+  if (!x1D || !VMapM || !VMapP || !halo_src_map || !normal_fn || !Escale || !Fscale)
+    return set_error(DGK_ERR_ARG, "null array argument");
+  int64_t Ne, NeA, NhaloNode;
+  DGK_TRY(dgk_mesh_sizes(NeX, NeY, NeZ, p, &Ne, &NeA, &NhaloNode));
+  const int Nq = p + 1, Np = Nq * Nq * Nq, Nfp = Nq * Nq, NfpTot = 6 * Nfp;
+  if ((int64_t)Np * NeA >= (int64_t)INT32_MAX)
+    return set_error(DGK_ERR_UNSUPPORTED, "mesh exceeds 2^31 nodes (int32 maps)");
+  const double h[3] = {dom_x / (double)NeX, dom_y / (double)NeY, dom_z / (double)NeZ};
+  double geo[kGeoStride];
+  fill_box_geo(h[0], h[1], h[2], geo);
+  int64_t ibuf = 0;
+  for (int k = 0; k < NeZ; ++k)
+    for (int j = 0; j < NeY; ++j)
+      for (int i = 0; i < NeX; ++i) {
+        const int64_t e = i + (int64_t)j * NeX + (int64_t)k * NeX * NeY;
+        const double c[3] = {((double)(i + 1) - 0.5) * h[0], ((double)(j + 1) - 0.5) * h[1],
+                             ((double)(k + 1) - 0.5) * h[2]};
+        for (int n = 0; n < Np; ++n) {
+          const int a[3] = {n % Nq, (n / Nq) % Nq, n / (Nq * Nq)};
+          for (int d = 0; d < 3; ++d) {
+            Escale[n + e * Np + (int64_t)d * Np * Ne] = geo[kGeoEscale + d];
+            if (pos_en)
+              pos_en[n + e * Np + (int64_t)d * Np * Ne] = c[d] + 0.5 * h[d] * x1D[a[d]];
+          }
+        }
+        for (int f = 0; f < 6; ++f) {
+          bool bnd;
+          const int64_t nb = neighbour(i, j, k, f, NeX, NeY, NeZ, bnd);
+          for (int fp = 0; fp < Nfp; ++fp) {
+            const int a = fp % Nq, b = fp / Nq;
+            const int64_t r = (int64_t)f * Nfp + fp + e * NfpTot;
+            Fscale[r] = geo[kGeoFscale + f];
+            for (int d = 0; d < 3; ++d)
+              normal_fn[r + (int64_t)d * NfpTot * Ne] = geo[kGeoNormal + 3 * f + d];
+            VMapM[r] = (int)(e * Np + own_face_node(f, a, b, Nq) + 1);
+            const int64_t src = nb * Np + nbr_face_node(f, a, b, Nq) + 1;
+            if (bnd) {
+              VMapP[r] = (int)((int64_t)Np * Ne + ibuf + 1);
+              halo_src_map[ibuf++] = (int)src;
+            } else {
+              VMapP[r] = (int)src;
+            }
+          }
+        }
+      }
+  return DGK_OK;
+}

@@ -1,0 +1,291 @@
+"""Python binding of libdgk.so (the C ABI declared in include/dgk.h).
+
+The binding is a thin ctypes layer: arrays cross the boundary as the
+reference's Fortran arrays (column-major float64 / int32), which map onto
+C-contiguous numpy arrays with the axes reversed, e.g. the reference's
+q(Np, Ne) is a numpy array of shape (Ne, Np).
+
+There is no CPU fallback: if libdgk.so has not been built, importing the
+library raises, and solver creation fails on a host without a HIP device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "libdgk.so"
+OPERATOR_DIR = PKG_DIR / "operator_data"
+
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_DEVICE, ERR_NOMEM, ERR_IO, ERR_STATE = range(7)
+EVAL_GENERAL, EVAL_OPT1, EVAL_HIP, EVAL_HIP_MP = 1, 2, 3, 4
+FIELD_Q, FIELD_U, FIELD_V, FIELD_W, FIELD_DQDT = range(5)
+TIMER_NAMES = ("Main", "Cal_tend", "Element boundary flux",
+               "Volume derivate + surface lift", "Stage (fused)")
+
+_P = C.c_void_p
+_I = C.c_int
+_L = C.c_int64
+_D = C.c_double
+_S = C.c_char_p
+
+# name -> (restype, argtypes); mirrors include/dgk.h
+_SIGNATURES = {
+    "dgk_last_error": (_S, []),
+    "dgk_version": (_I, []),
+    "dgk_device_count": (_I, [_P]),
+    "dgk_evaltype_id": (_I, [_S, _P]),
+    "dgk_element_sizes": (_I, [_I, _P, _P, _P, _P]),
+    "dgk_load_operators": (_I, [_S, _I, _P, _P, _P]),
+    "dgk_mesh_sizes": (_I, [_I, _I, _I, _I, _P, _P, _P]),
+    "dgk_mesh_build": (_I, [_I, _I, _I, _I, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
+    "dgk_create": (_I, [_I, _I, _L, _L, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "dgk_create_structured": (_I, [_I, _I, _I, _I, _I, _S, _D, _D, _D, _P]),
+    "dgk_destroy": (_I, [_P]),
+    "dgk_solver_info": (_I, [_P, _P, _P, _P, _P]),
+    "dgk_device_bytes": (_I, [_P, _P]),
+    "dgk_set_field": (_I, [_P, _I, _P, _L]),
+    "dgk_get_field": (_I, [_P, _I, _P, _L]),
+    "dgk_update_halo": (_I, [_P, _I]),
+    "dgk_copy_q0": (_I, [_P]),
+    "dgk_cal_elembnd_flux": (_I, [_P, _P]),
+    "dgk_cal_dqdt": (_I, [_P, _P]),
+    "dgk_cal_tend": (_I, [_P, _P]),
+    "dgk_rk_update": (_I, [_P, _I, _D]),
+    "dgk_step": (_I, [_P, _D, _I]),
+    "dgk_minmax": (_I, [_P, _P, _P]),
+    "dgk_synchronize": (_I, [_P]),
+    "dgk_set_kernel_timing": (_I, [_P, _I]),
+    "dgk_timer_report": (_I, [_P, _P, _P]),
+    "dgk_timer_reset": (_I, [_P]),
+}
+
+_lib = None
+
+
+class DgkError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dgk error {code}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Load libdgk.so (built in-tree by `make` in the package directory)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise DgkError(rc, lib().dgk_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous, "arrays must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().dgk_device_count(C.byref(n)))
+    return n.value
+
+
+def evaltype_id(name: str) -> int:
+    i = C.c_int(0)
+    _check(lib().dgk_evaltype_id(name.encode(), C.byref(i)))
+    return i.value
+
+
+def element_sizes(p: int) -> dict:
+    v = [C.c_int(0) for _ in range(4)]
+    _check(lib().dgk_element_sizes(p, *[C.byref(x) for x in v]))
+    return dict(zip(("Nq", "Np", "Nfp", "NfpTot"), (x.value for x in v)))
+
+
+def load_operators(p: int, directory: str | os.PathLike = OPERATOR_DIR):
+    """x1D (Nq,), D1D as numpy (Nq, Nq) with D1D[l, i] = D1D(i, l), and
+    Lift_mat as numpy (6, Nq, Nq, Nq) with Lift[f, k, j, i] = Lift_mat(i, j, k, f)."""
+    Nq = p + 1
+    x = np.zeros(Nq)
+    D = np.zeros((Nq, Nq))
+    L = np.zeros((6, Nq, Nq, Nq))
+    _check(lib().dgk_load_operators(str(directory).encode(), p, _ptr(x), _ptr(D), _ptr(L)))
+    return x, D, L
+
+
+def mesh_sizes(NeX: int, NeY: int, NeZ: int, p: int) -> dict:
+    v = [C.c_int64(0) for _ in range(3)]
+    _check(lib().dgk_mesh_sizes(NeX, NeY, NeZ, p, *[C.byref(x) for x in v]))
+    return dict(zip(("Ne", "NeA", "NhaloNode"), (x.value for x in v)))
+
+
+def mesh_build(NeX: int, NeY: int, NeZ: int, p: int, x1D: np.ndarray,
+               dom=(1.0, 1.0, 1.0), with_pos: bool = True) -> dict:
+    """The reference's mesh arrays (numpy axes reversed w.r.t. Fortran):
+    VMapM, VMapP (Ne, NfpTot) int32 1-based; halo_src_map (NhaloNode,);
+    normal_fn (3, Ne, NfpTot); Escale (3, Ne, Np); Fscale (Ne, NfpTot);
+    pos_en (3, Ne, Np)."""
+    es = element_sizes(p)
+    ms = mesh_sizes(NeX, NeY, NeZ, p)
+    Ne, NfpTot, Np = ms["Ne"], es["NfpTot"], es["Np"]
+    out = dict(
+        VMapM=np.zeros((Ne, NfpTot), np.int32),
+        VMapP=np.zeros((Ne, NfpTot), np.int32),
+        halo_src_map=np.zeros(ms["NhaloNode"], np.int32),
+        normal_fn=np.zeros((3, Ne, NfpTot)),
+        Escale=np.zeros((3, Ne, Np)),
+        Fscale=np.zeros((Ne, NfpTot)),
+        pos_en=np.zeros((3, Ne, Np)) if with_pos else None,
+    )
+    _check(lib().dgk_mesh_build(
+        NeX, NeY, NeZ, p, _ptr(_f64(x1D)), float(dom[0]), float(dom[1]), float(dom[2]),
+        _ptr(out["VMapM"]), _ptr(out["VMapP"]), _ptr(out["halo_src_map"]),
+        _ptr(out["normal_fn"]), _ptr(out["Escale"]), _ptr(out["Fscale"]),
+        _ptr(out["pos_en"])))
+    out.update(ms)
+    out.update(es)
+    return out
+
+
+class Solver:
+    """A device-resident advection solver (one HIP stream)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        Nq, Ne, NeA, NH = C.c_int(0), C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        _check(lib().dgk_solver_info(handle, C.byref(Nq), C.byref(Ne), C.byref(NeA), C.byref(NH)))
+        self.Nq, self.Ne, self.NeA, self.NhaloNode = Nq.value, Ne.value, NeA.value, NH.value
+        self.Np = self.Nq ** 3
+        self.NfpTot = 6 * self.Nq ** 2
+
+    @classmethod
+    def create(cls, evaltype, Nq, Ne, NeA, NhaloNode, D1D, Lift_mat, VMapM, VMapP,
+               halo_src_map, normal_fn, Escale, Fscale) -> "Solver":
+        """dgk_create from the reference's arrays (advect3d_eq_cal_tend inputs)."""
+        if isinstance(evaltype, str):
+            evaltype = evaltype_id(evaltype)
+        h = C.c_void_p()
+        keep = [_f64(D1D), _f64(Lift_mat), _i32(VMapM), _i32(VMapP), _i32(halo_src_map),
+                _f64(normal_fn), _f64(Escale), _f64(Fscale)]
+        _check(lib().dgk_create(evaltype, Nq, Ne, NeA, NhaloNode, *[_ptr(a) for a in keep],
+                                C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def structured(cls, evaltype, NeX, NeY, NeZ, p, dom=(1.0, 1.0, 1.0),
+                   operator_dir=OPERATOR_DIR) -> "Solver":
+        if isinstance(evaltype, str):
+            evaltype = evaltype_id(evaltype)
+        h = C.c_void_p()
+        _check(lib().dgk_create_structured(evaltype, NeX, NeY, NeZ, p,
+                                           str(operator_dir).encode(), float(dom[0]),
+                                           float(dom[1]), float(dom[2]), C.byref(h)))
+        return cls(h)
+
+    def close(self) -> None:
+        if self._h:
+            _check(lib().dgk_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # fields ---------------------------------------------------------------
+    def set_field(self, which: int, f: np.ndarray) -> None:
+        a = _f64(f).reshape(-1)
+        _check(lib().dgk_set_field(self._h, which, _ptr(a), a.size))
+
+    def get_field(self, which: int, with_halo: bool = False) -> np.ndarray:
+        ne = self.NeA if with_halo else self.Ne
+        out = np.zeros((ne, self.Np))
+        _check(lib().dgk_get_field(self._h, which, _ptr(out), out.size))
+        return out
+
+    def set_fields(self, q, u, v, w) -> None:
+        for which, f in ((FIELD_Q, q), (FIELD_U, u), (FIELD_V, v), (FIELD_W, w)):
+            self.set_field(which, f)
+
+    # reference stage pieces ----------------------------------------------
+    def update_halo(self, which: int = FIELD_Q) -> None:
+        _check(lib().dgk_update_halo(self._h, which))
+
+    def copy_q0(self) -> None:
+        _check(lib().dgk_copy_q0(self._h))
+
+    def cal_elembnd_flux(self, download: bool = True):
+        out = np.zeros((self.Ne, self.NfpTot)) if download else None
+        _check(lib().dgk_cal_elembnd_flux(self._h, _ptr(out)))
+        return out
+
+    def cal_dqdt(self, flux: np.ndarray | None = None) -> None:
+        a = None if flux is None else _f64(flux)
+        _check(lib().dgk_cal_dqdt(self._h, _ptr(a)))
+
+    def cal_tend(self, download: bool = True):
+        out = np.zeros((self.Ne, self.Np)) if download else None
+        _check(lib().dgk_cal_tend(self._h, _ptr(out)))
+        return out
+
+    def rk_update(self, stage: int, dt: float) -> None:
+        _check(lib().dgk_rk_update(self._h, stage, float(dt)))
+
+    def step(self, dt: float, nstep: int) -> None:
+        _check(lib().dgk_step(self._h, float(dt), int(nstep)))
+
+    def minmax(self):
+        a, b = C.c_double(0), C.c_double(0)
+        _check(lib().dgk_minmax(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def synchronize(self) -> None:
+        _check(lib().dgk_synchronize(self._h))
+
+    def device_bytes(self) -> int:
+        b = C.c_int64(0)
+        _check(lib().dgk_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    # timers ----------------------------------------------------------------
+    def set_kernel_timing(self, on: bool) -> None:
+        _check(lib().dgk_set_kernel_timing(self._h, int(bool(on))))
+
+    def timers(self) -> dict:
+        sec = np.zeros(5)
+        cnt = np.zeros(5, np.int64)
+        _check(lib().dgk_timer_report(self._h, _ptr(sec), _ptr(cnt)))
+        return {n: (float(s), int(c)) for n, s, c in zip(TIMER_NAMES, sec, cnt)}
+
+    def timer_reset(self) -> None:
+        _check(lib().dgk_timer_reset(self._h))

@@ -1,0 +1,182 @@
+"""Parity of the FP64 HIP path ("HIP") with the C oracle, through the C ABI.
+
+The oracle restates the reference GENERAL path and is bit-identical to the
+reference Fortran (tests/test_oracle_golden.py).  Tolerance for the FP64
+device path: max-norm relative error <= 1e-13 for one tendency evaluation and
+<= 1e-12 after a time loop (the FMA-fused device arithmetic differs from the
+reference's unfused products by rounding only).
+"""
+import numpy as np
+import pytest
+
+import dgk
+import oracle
+from dgk import inputs
+
+pytestmark = pytest.mark.gpu
+
+TEND_TOL = 1e-13
+LOOP_TOL = 1e-12
+
+
+def maxrel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def solver_from_mesh(m, evaltype="HIP", **over):
+    arr = dict(D1D=m.D, Lift_mat=m.Lift, VMapM=m.VMapM, VMapP=m.VMapP,
+               halo_src_map=m.halo_src_map, normal_fn=m.normal_fn, Escale=m.Escale,
+               Fscale=m.Fscale)
+    arr.update(over)
+    return dgk.Solver.create(evaltype, m.Nq, m.Ne, m.NeA, m.NhaloNode, **arr)
+
+
+def mesh_for(p):
+    return (3, 2, 2) if p <= 6 else (2, 2, 2) if p <= 11 else (2, 1, 2)
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+def test_cal_tend_every_order(p):
+    nx, ny, nz = mesh_for(p)
+    m = oracle.Mesh(nx, ny, nz, p)
+    q, u, v, w = inputs.random_fields(nx, ny, nz, p, seed=100 + p)
+    ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    with solver_from_mesh(m) as s:
+        s.set_fields(q, u, v, w)
+        got = s.cal_tend()
+    assert maxrel(got, ref) <= TEND_TOL
+
+
+@pytest.mark.parametrize("p", [1, 3, 7, 8, 15])
+def test_elembnd_flux_and_cal_dqdt(p):
+    nx, ny, nz = mesh_for(p)
+    m = oracle.Mesh(nx, ny, nz, p)
+    q, u, v, w = inputs.random_fields(nx, ny, nz, p, seed=200 + p)
+    Q, U, V, W = m.field(q), m.field(u), m.field(v), m.field(w)
+    flux_ref = m.elembnd_flux(Q, U, V, W)
+    with solver_from_mesh(m) as s:
+        s.set_fields(q, u, v, w)
+        flux = s.cal_elembnd_flux()
+        assert maxrel(flux, flux_ref) <= 1e-15
+        # cal_dqdt from an externally given flux
+        s.cal_dqdt(flux_ref)
+        dq = s.get_field(dgk.FIELD_DQDT)
+    ref = m.cal_dqdt(Q, U, V, W, flux_ref)
+    assert maxrel(dq, ref) <= TEND_TOL
+
+
+def test_update_halo_matches_reference_order():
+    m = oracle.Mesh(3, 2, 4, 4)
+    q, u, v, w = inputs.random_fields(3, 2, 4, 4, seed=5)
+    with solver_from_mesh(m) as s:
+        s.set_fields(q, u, v, w)
+        s.update_halo(dgk.FIELD_Q)
+        got = s.get_field(dgk.FIELD_Q, with_halo=True)
+        gu = s.get_field(dgk.FIELD_U, with_halo=True)
+    ref = m.field(q)
+    assert np.array_equal(got.reshape(-1)[: m.Np * m.Ne + m.NhaloNode],
+                          ref.reshape(-1)[: m.Np * m.Ne + m.NhaloNode])
+    assert np.array_equal(gu.reshape(-1)[: m.Np * m.Ne + m.NhaloNode],
+                          m.field(u).reshape(-1)[: m.Np * m.Ne + m.NhaloNode])
+
+
+def test_structured_equals_reference_arrays():
+    nx, ny, nz, p = 4, 3, 2, 5
+    m = oracle.Mesh(nx, ny, nz, p)
+    q, u, v, w = inputs.random_fields(nx, ny, nz, p, seed=9)
+    with solver_from_mesh(m) as a, dgk.Solver.structured("HIP", nx, ny, nz, p) as b:
+        a.set_fields(q, u, v, w)
+        b.set_fields(q, u, v, w)
+        assert np.array_equal(a.cal_tend(), b.cal_tend())
+        a.step(1e-3, 3)
+        b.step(1e-3, 3)
+        assert np.array_equal(a.get_field(dgk.FIELD_Q), b.get_field(dgk.FIELD_Q))
+
+
+def test_node_connectivity_twisted_faces():
+    """VMapP that does not pair faces node-for-node (z-boundary faces read the
+    transposed node order) runs through the per-node connectivity."""
+    nx, ny, nz, p = 2, 2, 3, 3
+    m = oracle.Mesh(nx, ny, nz, p)
+    Nq, Nfp = m.Nq, m.Nfp
+    VP = m.VMapP.copy()
+    perm = np.arange(Nfp).reshape(Nq, Nq).T.reshape(-1)
+    for e in range(m.Ne):
+        for f in (4, 5):
+            seg = VP[e, f * Nfp:(f + 1) * Nfp].copy()
+            VP[e, f * Nfp:(f + 1) * Nfp] = seg[perm]
+    m.VMapP = VP
+    q, u, v, w = inputs.random_fields(nx, ny, nz, p, seed=11)
+    ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    with solver_from_mesh(m) as s:
+        s.set_fields(q, u, v, w)
+        got = s.cal_tend()
+    assert maxrel(got, ref) <= TEND_TOL
+
+
+def test_per_element_geometry_records():
+    """Distinct Escale/Fscale per element (several geometry records)."""
+    nx, ny, nz, p = 3, 2, 2, 4
+    m = oracle.Mesh(nx, ny, nz, p)
+    scale = 1.0 + 0.25 * np.arange(m.Ne)
+    m.Escale = np.ascontiguousarray(m.Escale * scale[None, :, None])
+    m.Fscale = np.ascontiguousarray(m.Fscale * scale[:, None])
+    q, u, v, w = inputs.random_fields(nx, ny, nz, p, seed=12)
+    ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    with solver_from_mesh(m) as s:
+        s.set_fields(q, u, v, w)
+        got = s.cal_tend()
+    assert maxrel(got, ref) <= TEND_TOL
+
+
+def test_piecewise_stage_loop_equals_fused_step():
+    """copy_q0 / update_halo / cal_tend / rk_update (main.f90:51-75 piece by
+    piece) against dgk_step and the oracle loop."""
+    nx, ny, nz, p = 3, 3, 2, 3
+    m = oracle.Mesh(nx, ny, nz, p)
+    q, u, v, w = inputs.random_fields(nx, ny, nz, p, seed=13, vel_scale=0.5)
+    dt, nstep = 2e-3, 4
+    ref, _, _ = m.run(q, u, v, w, dt, nstep)
+    with solver_from_mesh(m) as s:
+        s.set_fields(q, u, v, w)
+        for _ in range(nstep):
+            s.copy_q0()
+            for stage in (1, 2, 3):
+                s.update_halo(dgk.FIELD_Q)
+                s.cal_tend(download=False)
+                s.rk_update(stage, dt)
+        piece = s.get_field(dgk.FIELD_Q)
+    with solver_from_mesh(m) as s:
+        s.set_fields(q, u, v, w)
+        s.step(dt, nstep)
+        fused = s.get_field(dgk.FIELD_Q)
+    assert maxrel(piece, ref) <= LOOP_TOL
+    assert maxrel(fused, ref) <= LOOP_TOL
+
+
+def test_c1_full_run_matches_oracle():
+    wl = inputs.c1_sine_noise()
+    m = oracle.Mesh(wl.NeX, wl.NeY, wl.NeZ, wl.p)
+    ref, mm_ref, _ = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, wl.nstep, trace=True)
+    with dgk.Solver.structured("HIP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        mm = []
+        for _ in range(wl.nstep):
+            s.step(wl.dt, 1)
+            mm.append(s.minmax())
+        got = s.get_field(dgk.FIELD_Q)
+    assert maxrel(got, ref) <= LOOP_TOL
+    assert np.abs(np.array(mm) - mm_ref).max() <= 1e-12 * np.abs(mm_ref).max()
+
+
+@pytest.mark.parametrize("nstep", [3])
+def test_c2_full_size_short_run(nstep):
+    """32^3 elements at p=7 (16.8M DOF): a few steps against the oracle."""
+    wl = inputs.c2_gaussian(nstep=nstep)
+    m = oracle.Mesh(wl.NeX, wl.NeY, wl.NeZ, wl.p)
+    ref, _, _ = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, nstep)
+    with dgk.Solver.structured("HIP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        s.step(wl.dt, nstep)
+        got = s.get_field(dgk.FIELD_Q)
+    assert maxrel(got, ref) <= LOOP_TOL
