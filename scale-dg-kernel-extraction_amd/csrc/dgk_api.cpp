@@ -46,6 +46,9 @@ struct dgk_solver {
   bool flux_valid = false, dqdt_valid = false, q0_valid = false;
   float *u32 = nullptr, *v32 = nullptr, *w32 = nullptr;
   float *D32 = nullptr, *ell32 = nullptr;
+  double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
+  int32_t *order = nullptr;  // element processing order
+  int axis_aligned = 0;
   int32_t *nbr = nullptr, *vmapP = nullptr, *halo = nullptr;
   double *geo = nullptr, *D = nullptr, *Lift = nullptr;
   uint32_t *geoid = nullptr;
@@ -176,8 +179,8 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
   StageArgs a;
   a.q = qin; a.q0 = q0; a.u = s->u; a.v = s->v; a.w = s->w; a.out = out;
   a.flux_in = nullptr; a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D; a.Lift = s->Lift;
-  a.geo = s->geo; a.geoid = s->geoid; a.Ne = s->Ne;
-  a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode;
+  a.ell = s->ell; a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.Ne = s->Ne;
+  a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
   return a;
 }
 
@@ -187,23 +190,24 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
     StageArgsMP a;
     a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
     a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
-    a.geo = s->geo; a.geoid = s->geoid; a.Ne = s->Ne;
-    a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode;
+    a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.Ne = s->Ne;
+    a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
     return launch_stage_mp(s->Nq, a, s->stream);
   }
   return launch_stage_f64(s->Nq, stage_args(s, qin, q0, out, mode, ca, cb, dt), s->stream);
 }
 
-// Separable 1D form of the stored Lift for the mixed-precision path:
-// Lift(:,:,:,f) = I (x) l(-/+) (x) I along the face normal.  Taken from the
-// face-2 (+x) and face-4 (-x) blocks at tangential node (0,0); every block
-// must agree with it to 1e-10 relative.
-int separable_lift(const MeshTables &t, std::vector<float> &ell) {
+// Separable 1D form of the stored Lift: Lift(:,:,:,f) = I (x) l-/+ (x) I
+// along the face normal (faces -y,+x,+y,-x,-z,+z use l-(j), l+(i), l+(j),
+// l-(i), l-(k), l+(k)).  l-/+ are read from the -x and +x blocks at
+// tangential node (0,0); returns the largest deviation of any stored entry
+// from that form relative to max|Lift|.
+double separable_lift(const MeshTables &t, std::vector<double> &ell) {
   const int Nq = t.Nq, Np = t.Np;
-  std::vector<double> lm(Nq), lp(Nq);
+  ell.assign(2 * Nq, 0.0);
   for (int a = 0; a < Nq; ++a) {
-    lp[a] = t.Lift[a + 1 * Np];  // face +x: Lift(a,0,0,2) = l+(a)
-    lm[a] = t.Lift[a + 3 * Np];  // face -x: Lift(a,0,0,4) = l-(a)
+    ell[a] = t.Lift[a + 3 * Np];       // face -x: Lift(a,0,0,4) = l-(a)
+    ell[Nq + a] = t.Lift[a + 1 * Np];  // face +x: Lift(a,0,0,2) = l+(a)
   }
   double scale = 0.0;
   for (double x : t.Lift) scale = std::fmax(scale, std::fabs(x));
@@ -214,20 +218,17 @@ int separable_lift(const MeshTables &t, std::vector<float> &ell) {
         for (int i = 0; i < Nq; ++i) {
           const int axis_idx = (f == 1 || f == 3) ? i : (f == 0 || f == 2) ? j : k;
           const bool plus = (f == 1 || f == 2 || f == 5);
-          const double ref = plus ? lp[axis_idx] : lm[axis_idx];
+          const double ref = ell[(plus ? Nq : 0) + axis_idx];
           dev = std::fmax(dev, std::fabs(t.Lift[i + j * Nq + k * Nq * Nq + f * Np] - ref));
         }
-  if (dev > 1e-10 * scale)
-    return set_error(DGK_ERR_UNSUPPORTED,
-                     "Lift_mat is not separable (deviation " + std::to_string(dev) +
-                         "); HIP_MP needs the tensor-product Lift");
-  ell.resize(2 * Nq);
-  for (int a = 0; a < Nq; ++a) {
-    ell[a] = (float)lm[a];
-    ell[Nq + a] = (float)lp[a];
-  }
-  return DGK_OK;
+  return scale > 0.0 ? dev / scale : dev;
 }
+
+// Lift_mat deviations from the tensor-product form below this (relative)
+// are rounding in the stored data (2.8e-14 for p = 1..15); the FP64 path then
+// applies the 1D form, otherwise it reads Lift_mat as stored.
+constexpr double kSeparableF64 = 1e-12;
+constexpr double kSeparableMP = 1e-10;
 
 int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   if (!out) return set_error(DGK_ERR_ARG, "output handle pointer is NULL");
@@ -242,8 +243,13 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
     (void)hipGetLastError();
     return set_error(DGK_ERR_DEVICE, "no HIP device available (the path has no CPU fallback)");
   }
-  std::vector<float> ell;
-  if (evaltype == DGK_EVAL_HIP_MP) DGK_TRY(separable_lift(t, ell));
+  std::vector<double> ell;
+  const double sep_dev = separable_lift(t, ell);
+  if (evaltype == DGK_EVAL_HIP_MP && !(sep_dev <= kSeparableMP))
+    return set_error(DGK_ERR_UNSUPPORTED,
+                     "Lift_mat is not of tensor-product form (relative deviation " +
+                         std::to_string(sep_dev) + "); HIP_MP needs the separable Lift");
+  const bool use_ell = sep_dev <= kSeparableF64;
 
   auto *s = new dgk_solver();
   auto fail = [&](int rc) {
@@ -256,6 +262,7 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   s->Nq = t.Nq; s->Np = t.Np; s->Nfp = t.Nfp; s->NfpTot = t.NfpTot;
   s->Ne = t.Ne; s->NeA = t.NeA; s->NhaloNode = t.NhaloNode;
   s->nfield = (int64_t)t.Np * t.Ne + t.NhaloNode;
+  s->axis_aligned = t.axis_aligned ? 1 : 0;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(DGK_ERR_DEVICE, "hipStreamCreate failed"));
   for (int b = 0; b < 3; ++b)
@@ -270,7 +277,8 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
       (rc = dalloc(s, &s->geoid, (int64_t)t.geoid.size())) ||
       (rc = dalloc(s, &s->D, (int64_t)t.D.size())) ||
       (rc = dalloc(s, &s->Lift, (int64_t)t.Lift.size())) ||
-      (rc = dalloc(s, &s->mm, 2 * kMinmaxBlocks + 2)))
+      (rc = dalloc(s, &s->mm, 2 * kMinmaxBlocks + 2)) ||
+      (rc = dalloc(s, &s->order, (int64_t)t.order.size())))
     return fail(rc);
   if (evaltype == DGK_EVAL_HIP_MP) {
     const int64_t n = (int64_t)t.Np * t.Ne;
@@ -278,13 +286,17 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
         (rc = dalloc(s, &s->w32, n)) || (rc = dalloc(s, &s->D32, (int64_t)t.D.size())) ||
         (rc = dalloc(s, &s->ell32, (int64_t)ell.size())))
       return fail(rc);
-    std::vector<float> D32(t.D.begin(), t.D.end());
-    if ((rc = upload(s, s->D32, D32)) || (rc = upload(s, s->ell32, ell))) return fail(rc);
+    std::vector<float> D32(t.D.begin(), t.D.end()), ell32(ell.begin(), ell.end());
+    if ((rc = upload(s, s->D32, D32)) || (rc = upload(s, s->ell32, ell32))) return fail(rc);
+  }
+  if (use_ell) {
+    if ((rc = dalloc(s, &s->ell, (int64_t)ell.size())) || (rc = upload(s, s->ell, ell)))
+      return fail(rc);
   }
   if ((rc = upload(s, s->nbr, t.nbr)) || (rc = upload(s, s->vmapP, t.vmapP)) ||
       (rc = upload(s, s->halo, t.halo_src)) || (rc = upload(s, s->geo, t.geo)) ||
       (rc = upload(s, s->geoid, t.geoid)) || (rc = upload(s, s->D, t.D)) ||
-      (rc = upload(s, s->Lift, t.Lift)))
+      (rc = upload(s, s->Lift, t.Lift)) || (rc = upload(s, s->order, t.order)))
     return fail(rc);
   for (int b = 0; b < 3; ++b)
     if (hipMemsetAsync(s->q[b], 0, sizeof(double) * s->nfield, s->stream) != hipSuccess)

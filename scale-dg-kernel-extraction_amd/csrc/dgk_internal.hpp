@@ -81,7 +81,21 @@ struct MeshTables {
   std::vector<uint32_t> geoid;    // Ne, empty when ngeo == 1
   std::vector<double> D;          // Nq*Nq column-major
   std::vector<double> Lift;       // Nq^3*6 column-major
+  std::vector<int32_t> order;     // element processing order (empty: identity)
+  bool axis_aligned = false;      // every face normal lies along its face axis
 };
+
+// Normal axis of faces -y, +x, +y, -x, -z, +z (set_face_geometry,
+// mod_mesh.f90:364-394).
+constexpr int kFaceAxis[6] = {1, 0, 1, 0, 2, 2};
+// True when every geometry record's face normals have zero components off
+// the face's axis.
+bool geo_axis_aligned(const std::vector<double> &geo);
+
+// Processing order of a NeX x NeY x NeZ box in bricks of bx x by x bz
+// elements (bricks x-fastest, elements x-fastest inside a brick).  Only the
+// assignment of elements to workgroups changes, never the arithmetic.
+std::vector<int32_t> brick_order(int NeX, int NeY, int NeZ, int bx, int by, int bz);
 
 int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
                                 int64_t NhaloNode, const double *D1D,
@@ -108,12 +122,15 @@ struct StageArgs {
   const int32_t *nbr;     // compact connectivity or nullptr
   const int32_t *vmapP;   // node connectivity or nullptr
   const double *D;        // Nq*Nq
-  const double *Lift;     // Nq^3*6
+  const double *Lift;     // Nq^3*6 (stored-Lift kernel)
+  const double *ell;      // 2*Nq separable Lift l-, l+ (nullptr: stored Lift)
   const double *geo;
   const uint32_t *geoid;  // nullptr when one record
+  const int32_t *order;   // processing order of elements (nullptr: identity)
   int64_t Ne;
   double ca, cb, dt;      // RK stage: out = ca*q0 + cb*(q + dt*tend)
   int mode;               // 0 = tendency, 1 = RK stage
+  int axis_aligned;       // face normals along the face axes (see MeshTables)
 };
 
 // Mixed-precision operands (FP32 copies of the constant inputs).
@@ -128,9 +145,11 @@ struct StageArgsMP {
   const float *ell;       // 2*Nq: l-(x) then l+(x), the separable 1D Lift
   const double *geo;
   const uint32_t *geoid;
+  const int32_t *order;
   int64_t Ne;
   double ca, cb, dt;
   int mode;
+  int axis_aligned;
 };
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);

@@ -1,22 +1,33 @@
 // FP64 device kernels of the DG advection path (gfx950).
 //
-// stage_f64<NQ,EPB>: one RK stage of the whole mesh in one launch.  Each
-// element is owned by NQ*NQ threads; thread (i,j) holds the k-column of its
-// element in registers (coalesced: a k-slab of one element is NQ*NQ
-// consecutive doubles).  Per element:
-//   A. load q,u,v,w columns; F = q*(u,v,w); Fx slab-transposed into LDS;
-//      the element's own face traces (q, normal velocity) into LDS;
-//   B. upwind flux of face node (i,j) of all six faces (neighbour traces are
-//      gathered from HBM/L2 through the folded neighbour map), and the
-//      x-derivative from LDS;
-//   C. Fy into LDS, y-derivative, z-derivative in registers, Lift from the
-//      stored Lift_mat, dqdt = -(((Ex Dx + Ey Dy) + Ez Dz) + Lift), then either
-//      store dqdt or the RK stage value a*q0 + b*(q + dt*dqdt).
-// This fuses cal_elembnd_flux (mod_advect3d_eq.f90:89-142), cal_dqdt
-// (mod_advect3d_eq.f90:146-201), tensorprod_divlike_dirXYZ_general and
-// tensorprod_Lift_hexahedral_general (mod_dg_optr_kernel.f90:197-288) and the
-// RK update (main.f90:69-73).  The contraction order is the reference's
+// One RK stage of the whole mesh is one launch.  The elements of a launch are
+// independent: an element reads its own nodes, its neighbours' face traces of
+// the stage input, and writes only its own nodes.  Both stage kernels fuse
+// cal_elembnd_flux (mod_advect3d_eq.f90:89-142), cal_dqdt
+// (mod_advect3d_eq.f90:146-201) with tensorprod_divlike_dirXYZ and
+// tensorprod_Lift_hexahedral (mod_dg_optr_kernel.f90:197-288), and the RK
+// update (main.f90:69-73).  Sums over l run in the reference's order
 // (l ascending); products are fused into FMAs.
+//
+// stage_f64<NQ> (separable Lift): NQ*NQ lanes per element, lane (i,j) owns
+// the k-column (i,j,:).  At p <= 7 a workgroup is one wavefront holding
+// 64/(NQ*NQ) elements; above, an element spans ceil(NQ*NQ/64) wavefronts.
+//   1. the workgroup's q,u,v,w blocks are copied HBM -> LDS with
+//      global_load_lds_dwordx4 (no VGPRs held, 1 KiB per wave instruction);
+//      meanwhile each lane gathers the neighbour traces of its face node (i,j)
+//      on all six faces into registers;
+//   2. upwind flux of face node (i,j) of every face from the LDS copy;
+//   3. F = q*(u,v,w) in place in LDS;
+//   4. per node: the three sum-factorised contractions from LDS (D rows in
+//      registers), the Lift in tensor-product form -- face f contributes
+//      l-/+(x_n) * flux_f(tangential node) along its normal n, with l-/+ taken
+//      from the stored Lift_mat (checked separable at solver creation) -- and
+//      dqdt = -(((Ex Dx + Ey Dy) + Ez Dz) + Lift), stored or RK-combined.
+//
+// stage_f64_lift<NQ> (stored Lift): the same arithmetic with Lift_mat read as
+// stored; used when Lift_mat is not of tensor-product form.
+#include <algorithm>
+
 #include "dgk_internal.hpp"
 
 namespace dgk {
@@ -31,6 +42,233 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
   return x * per + min(x, rem) + local;
 }
 
+// Broadcast lane 0's value: marks a value the compiler cannot prove uniform
+// as wave-uniform, so loads addressed by it become scalar loads.
+__device__ __forceinline__ int64_t uniform(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+template <int NQ>
+struct Staged {
+  static constexpr int NQ2 = NQ * NQ;
+  static constexpr int NP = NQ2 * NQ;
+  static constexpr int EPB = NQ2 <= 64 ? 64 / NQ2 : 1;  // elements per workgroup
+  static constexpr int THREADS = NQ2 <= 64 ? 64 : ((NQ2 + 63) / 64) * 64;
+  static constexpr int WAVES = THREADS / 64;
+  // Async copy granule: 16 B per lane when every element block is 16-byte
+  // aligned (NP even), else 4 B per lane.
+  static constexpr bool VEC16 = (NP % 2) == 0;
+  static constexpr int CHUNK = VEC16 ? 128 : 32;  // doubles per wave instruction
+  static constexpr int NCHUNK = (EPB * NP + CHUNK - 1) / CHUNK;
+  static constexpr int FIELD = NCHUNK * CHUNK;    // padded staged field (doubles)
+  static constexpr int OFF_G = 4 * FIELD;         // x/y-face fluxes [EPB][4][NQ2]
+  static constexpr int OFF_D = OFF_G + EPB * 4 * NQ2;
+  static constexpr int OFF_L = OFF_D + NQ2;
+  static constexpr int LDS_DOUBLES = OFF_L + 2 * NQ;
+  static constexpr size_t LDS_BYTES = sizeof(double) * LDS_DOUBLES;
+  // In a one-wavefront workgroup, Fx can be written slab-transposed in place
+  // (lane (i,j) writes where lane (j,i) read in the same instruction slot);
+  // the x-contraction then reads consecutive addresses.
+  static constexpr bool FX_T = WAVES == 1;
+};
+
+template <int NQ>
+__global__ void __launch_bounds__(Staged<NQ>::THREADS)
+stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
+          const double *__restrict__ v_, const double *__restrict__ w_, double *out_,
+          const double *__restrict__ flux_in_, const int32_t *__restrict__ nbr_,
+          const int32_t *__restrict__ vmapP_, const double *__restrict__ D_,
+          const double *__restrict__ ell_, const double *__restrict__ geo_,
+          const uint32_t *__restrict__ geoid_, const int32_t *__restrict__ order_,
+          const int64_t Ne_, const double ca_, const double cb_, const double dt_,
+          const int mode_, const int axis_aligned_) {
+  // Read-only operands are __restrict__ kernel parameters, so wave-uniform
+  // reads of them (operators, geometry, neighbour table) are scalar loads.
+  using C = Staged<NQ>;
+  constexpr int NQ2 = C::NQ2, NP = C::NP, EPB = C::EPB, NFT = 6 * NQ2;
+  extern __shared__ double smem[];
+  double *sq = smem, *su = smem + C::FIELD, *sv = smem + 2 * C::FIELD,
+         *sw = smem + 3 * C::FIELD;
+  double *sG = smem + C::OFF_G, *sD = smem + C::OFF_D, *sL = smem + C::OFF_L;
+
+  const int tid = threadIdx.x;
+  const int64_t slot0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB;
+
+  // ---- 1. async HBM -> LDS copy of q, u, v, w -------------------------------
+  {
+    const int wave = tid >> 6, lane = tid & 63;
+    const double *src[4] = {q_, u_, v_, w_};
+#pragma unroll
+    for (int c0 = 0; c0 < C::NCHUNK; c0 += C::WAVES) {
+      const int c = c0 + wave;
+      if (c < C::NCHUNK) {
+        const int d = c * C::CHUNK + (C::VEC16 ? 2 * lane : lane / 2);  // staged double
+        const int del = d / NP, dn = d - del * NP;
+        const int64_t sl = slot0 + del;
+        int64_t off = 0;
+        if (del < EPB && sl < Ne_) off = (order_ ? (int64_t)order_[sl] : sl) * NP + dn;
+#pragma unroll
+        for (int fld = 0; fld < 4; ++fld) {
+          const char *gp = (const char *)(src[fld] + off) + (C::VEC16 ? 0 : 4 * (lane & 1));
+          __attribute__((address_space(3))) void *lp =
+              (__attribute__((address_space(3))) void *)(smem + fld * C::FIELD + c * C::CHUNK);
+#if defined(__HIP_DEVICE_COMPILE__)
+          __builtin_amdgcn_global_load_lds((const void *)gp, lp, C::VEC16 ? 16 : 4, 0, 0);
+#else
+          (void)gp;
+          (void)lp;
+#endif
+        }
+      }
+    }
+  }
+  for (int t = tid; t < NQ2; t += C::THREADS) sD[t] = D_[t];
+  for (int t = tid; t < 2 * NQ; t += C::THREADS) sL[t] = ell_[t];
+
+  const bool lane_ok = tid < EPB * NQ2;
+  const int el = lane_ok ? tid / NQ2 : 0;
+  const int ij = lane_ok ? tid - el * NQ2 : 0;
+  const int i = ij % NQ, j = ij / NQ;
+  const int64_t slot = slot0 + el;
+  const bool active = lane_ok && slot < Ne_;
+  int64_t e;
+  const double *g;
+  if (EPB == 1) {
+    // one element per workgroup: its index, neighbour table entries and
+    // geometry record are wave-uniform and read through the scalar cache
+    e = slot < Ne_ ? (order_ ? (int64_t)order_[slot] : slot) : 0;
+    e = uniform(e);
+    g = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+  } else {
+    e = active ? (order_ ? (int64_t)order_[slot] : slot) : 0;
+    g = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+  }
+  const int64_t base = e * NP + ij;
+  const int lb = el * NP;  // element block in the staged fields
+  const bool rk = mode_ == 1;
+  const bool use_q0 = rk && ca_ != 0.0;
+
+  // neighbour traces of face node (i,j) on all six faces, and the RK base
+  // column, in flight together with the async copy
+  // With axis-aligned face normals only the normal velocity component of the
+  // neighbour is gathered (the others enter the normal velocity as exact
+  // zeros); the component is fixed per face at compile time.
+  double qP[6], uP[6], vP[6], wP[6], q0v[NQ];
+  const bool axal = axis_aligned_ != 0;
+  if (active && !flux_in_) {
+    if (axal) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int64_t pidx = vmapP_ ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
+                                     : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+        const double *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
+        qP[f] = q_[pidx];
+        uP[f] = vel[pidx];
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int64_t pidx = vmapP_ ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
+                                     : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+        qP[f] = q_[pidx];
+        uP[f] = u_[pidx];
+        vP[f] = v_[pidx];
+        wP[f] = w_[pidx];
+      }
+    }
+  }
+  if (active && use_q0) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q0v[k] = q0_[base + (int64_t)k * NQ2];
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
+  __syncthreads();
+
+  // ---- 2. upwind flux of face node (i,j) on each face ----------------------
+  double gf[6];
+  if (active) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      if (flux_in_) {
+        gf[f] = flux_in_[e * NFT + f * NQ2 + ij];
+      } else {
+        const double n1 = g[kGeoNormal + 3 * f], n2 = g[kGeoNormal + 3 * f + 1],
+                     n3 = g[kGeoNormal + 3 * f + 2];
+        const int m = lb + own_face_node(f, i, j, NQ);
+        const double qM = sq[m];
+        double VelM, VelP;
+        if (axal) {
+          const double *sa = kFaceAxis[f] == 0 ? su : kFaceAxis[f] == 1 ? sv : sw;
+          const double na = g[kGeoNormal + 3 * f + kFaceAxis[f]];
+          VelM = sa[m] * na;
+          VelP = uP[f] * na;
+        } else {
+          VelM = su[m] * n1 + sv[m] * n2 + sw[m] * n3;
+          VelP = uP[f] * n1 + vP[f] * n2 + wP[f] * n3;
+        }
+        const double alpha = 0.5 * fabs(VelP + VelM);
+        gf[f] = 0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qM * VelM - alpha * (qP[f] - qM));
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) sG[(el * 4 + f) * NQ2 + ij] = gf[f];
+  }
+  __syncthreads();
+
+  // ---- 3. volume fluxes: Fx, Fy in LDS, Fz and q in registers ---------------
+  double qc[NQ], fz[NQ];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int n = lb + ij + k * NQ2;
+      const double qq = sq[n], uu = su[n], vv = sv[n], ww = sw[n];
+      qc[k] = qq;
+      fz[k] = qq * ww;
+      if (C::FX_T) su[lb + j + i * NQ + k * NQ2] = qq * uu;
+      else su[n] = qq * uu;
+      sv[n] = qq * vv;
+    }
+  }
+  __syncthreads();
+
+  // ---- 4. contractions, Lift, combine, epilogue -----------------------------
+  if (active) {
+    double Di[NQ], Dj[NQ];
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) {
+      Di[l] = sD[i + l * NQ];
+      Dj[l] = sD[j + l * NQ];
+    }
+    const double Ex = g[kGeoEscale], Ey = g[kGeoEscale + 1], Ez = g[kGeoEscale + 2];
+    const double lmi = sL[i], lpi = sL[NQ + i], lmj = sL[j], lpj = sL[NQ + j];
+    const double *G = sG + el * 4 * NQ2;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      double dx = 0.0, dy = 0.0, dz = 0.0;
+#pragma unroll
+      for (int l = 0; l < NQ; ++l) {
+        dx += Di[l] * (C::FX_T ? su[lb + j + l * NQ + k * NQ2] : su[lb + l + j * NQ + k * NQ2]);
+        dy += Dj[l] * sv[lb + i + l * NQ + k * NQ2];
+        dz += D_[k + l * NQ] * fz[l];
+      }
+      const double lift = lmj * G[0 * NQ2 + i + k * NQ] + lpi * G[1 * NQ2 + j + k * NQ] +
+                          lpj * G[2 * NQ2 + i + k * NQ] + lmi * G[3 * NQ2 + j + k * NQ] +
+                          sL[k] * gf[4] + sL[NQ + k] * gf[5];
+      const double tend = -(((Ex * dx + Ey * dy) + Ez * dz) + lift);
+      const int64_t idx = base + (int64_t)k * NQ2;
+      if (!rk) {
+        out_[idx] = tend;
+      } else {
+        const double upd = cb_ * (qc[k] + dt_ * tend);
+        out_[idx] = use_q0 ? ca_ * q0v[k] + upd : upd;
+      }
+    }
+  }
+}
+
+// Stored-Lift kernel: Lift_mat read as stored (any Lift_mat).
 template <int NQ>
 struct Tile {
   static constexpr int NQ2 = NQ * NQ;
@@ -42,7 +280,7 @@ struct Tile {
 
 template <int NQ>
 __global__ void __launch_bounds__(Tile<NQ>::THREADS)
-stage_f64(const StageArgs P) {
+stage_f64_lift(const StageArgs P) {
   constexpr int NQ2 = Tile<NQ>::NQ2, NP = Tile<NQ>::NP, EPB = Tile<NQ>::EPB;
   constexpr int NFT = 6 * NQ2;
   __shared__ double sF[EPB][NP];        // Fx (slab-transposed), then Fy
@@ -52,8 +290,9 @@ stage_f64(const StageArgs P) {
   const int tid = threadIdx.x;
   const int el = tid / NQ2, ij = tid - el * NQ2;
   const int i = ij % NQ, j = ij / NQ;
-  const int64_t e = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB + el;
-  const bool active = e < P.Ne;
+  const int64_t slot = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB + el;
+  const bool active = slot < P.Ne;
+  const int64_t e = active ? (P.order ? (int64_t)P.order[slot] : slot) : 0;
 
   const double *g = P.geo + (active && P.geoid ? (int64_t)P.geoid[e] * kGeoStride : 0);
   const int64_t base = e * NP + ij;
@@ -166,10 +405,25 @@ stage_f64(const StageArgs P) {
 
 template <int NQ>
 int launch_nq(const StageArgs &a, hipStream_t s) {
-  using T = Tile<NQ>;
-  const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
-  if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  hipLaunchKernelGGL(stage_f64<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), 0, s, a);
+  if (a.ell) {
+    using T = Staged<NQ>;
+    const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
+    if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
+    static bool attr_set = false;
+    if (!attr_set) {
+      DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)T::LDS_BYTES));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(stage_f64<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
+                       a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
+                       a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned);
+  } else {
+    using T = Tile<NQ>;
+    const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
+    if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
+    hipLaunchKernelGGL(stage_f64_lift<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), 0, s, a);
+  }
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }

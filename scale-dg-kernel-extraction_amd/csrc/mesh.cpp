@@ -9,7 +9,9 @@
 //                          read: a 6-entry neighbour table per element (or a
 //                          resolved per-face-node index), the halo folded into
 //                          it, and deduplicated per-element geometry records.
+#include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -42,8 +44,7 @@ bool read_record(std::ifstream &in, double *dst, size_t n) {
   return true;
 }
 
-// Reference face order -y, +x, +y, -x, -z, +z: normal axis and sign.
-constexpr int kFaceAxis[6] = {1, 0, 1, 0, 2, 2};
+// Reference face order -y, +x, +y, -x, -z, +z: normal sign (axis: kFaceAxis).
 constexpr double kFaceSign[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
 
 // Periodic neighbour of element (i,j,k) across face f (0-based indices,
@@ -82,6 +83,30 @@ struct RecHash {
 };
 
 }  // namespace
+
+bool geo_axis_aligned(const std::vector<double> &geo) {
+  for (size_t r = 0; r + kGeoStride <= geo.size(); r += kGeoStride)
+    for (int f = 0; f < 6; ++f)
+      for (int d = 0; d < 3; ++d)
+        if (d != kFaceAxis[f] && geo[r + kGeoNormal + 3 * f + d] != 0.0) return false;
+  return true;
+}
+
+std::vector<int32_t> brick_order(int NeX, int NeY, int NeZ, int bx, int by, int bz) {
+  bx = std::max(1, std::min(bx, NeX));
+  by = std::max(1, std::min(by, NeY));
+  bz = std::max(1, std::min(bz, NeZ));
+  std::vector<int32_t> order;
+  order.reserve((size_t)NeX * NeY * NeZ);
+  for (int K = 0; K < NeZ; K += bz)
+    for (int J = 0; J < NeY; J += by)
+      for (int I = 0; I < NeX; I += bx)
+        for (int k = K; k < std::min(K + bz, NeZ); ++k)
+          for (int j = J; j < std::min(J + by, NeY); ++j)
+            for (int i = I; i < std::min(I + bx, NeX); ++i)
+              order.push_back((int32_t)(i + (int64_t)j * NeX + (int64_t)k * NeX * NeY));
+  return order;
+}
 
 int load_operators(const char *dir, int p, std::vector<double> &x1D,
                    std::vector<double> &D, std::vector<double> &Lift) {
@@ -145,6 +170,12 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
   fill_box_geo(dom_x / (double)NeX, dom_y / (double)NeY, dom_z / (double)NeZ,
                t.geo.data());
   t.geoid.clear();
+  t.order.clear();
+  int bx = 0, by = 0, bz = 0;
+  if (const char *env = std::getenv("DGK_ORDER"))
+    if (std::sscanf(env, "brick:%d,%d,%d", &bx, &by, &bz) == 3 && bx > 0 && by > 0 && bz > 0)
+      t.order = brick_order(NeX, NeY, NeZ, bx, by, bz);
+  t.axis_aligned = geo_axis_aligned(t.geo);
   return DGK_OK;
 }
 
@@ -267,6 +298,7 @@ int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
   t.geo.swap(geo);
   if (ids.size() == 1) t.geoid.clear();
   else t.geoid.swap(geoid);
+  t.axis_aligned = geo_axis_aligned(t.geo);
   return DGK_OK;
 }
 

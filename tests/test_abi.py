@@ -1,0 +1,104 @@
+"""The C ABI of libdgk.so without a GPU: every declared symbol is exported,
+host-side helpers work, and invalid inputs fail with the documented status
+codes (CPU).  Solver creation on a host without a HIP device must fail
+loudly (DGK_ERR_DEVICE): there is no CPU fallback."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import dgk
+import oracle
+from dgk import inputs
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "dgk.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dgk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 25
+    lib = C.CDLL(str(dgk.LIB_PATH))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_the_header():
+    assert set(declared_functions()) == set(dgk._SIGNATURES)
+
+
+def test_evaltype_names():
+    assert dgk.evaltype_id("GENERAL") == dgk.EVAL_GENERAL
+    assert dgk.evaltype_id("OPT1    ") == dgk.EVAL_OPT1  # Fortran blank padding
+    assert dgk.evaltype_id("HIP") == dgk.EVAL_HIP
+    assert dgk.evaltype_id("HIP_MP") == dgk.EVAL_HIP_MP
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.evaltype_id("CUDA")
+    assert ei.value.code == dgk.ERR_ARG and "Unsupported dgoptr_evaltype" in str(ei.value)
+
+
+def test_sizes_and_operator_errors():
+    assert dgk.element_sizes(7) == dict(Nq=8, Np=512, Nfp=64, NfpTot=384)
+    assert dgk.mesh_sizes(32, 32, 32, 7) == dict(Ne=32768, NeA=38912, NhaloNode=393216)
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.element_sizes(16)
+    assert ei.value.code == dgk.ERR_UNSUPPORTED
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.load_operators(3, "/nonexistent")
+    assert ei.value.code == dgk.ERR_IO
+
+
+def _arrays(m, **over):
+    a = dict(D1D=m.D, Lift_mat=m.Lift, VMapM=m.VMapM, VMapP=m.VMapP,
+             halo_src_map=m.halo_src_map, normal_fn=m.normal_fn, Escale=m.Escale,
+             Fscale=m.Fscale)
+    a.update(over)
+    return a
+
+
+def test_create_validates_reference_arrays():
+    m = oracle.Mesh(2, 2, 2, 2)
+    bad = m.VMapM.copy()
+    bad[0, 0] += 1
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.Solver.create("HIP", m.Nq, m.Ne, m.NeA, m.NhaloNode, **_arrays(m, VMapM=bad))
+    assert ei.value.code == dgk.ERR_UNSUPPORTED
+    badp = m.VMapP.copy()
+    badp[0, 0] = m.Np * m.NeA + 5
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.Solver.create("HIP", m.Nq, m.Ne, m.NeA, m.NhaloNode, **_arrays(m, VMapP=badp))
+    assert ei.value.code == dgk.ERR_ARG
+    esc = m.Escale.copy()
+    esc[0, 0, 1] *= 1.5
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.Solver.create("HIP", m.Nq, m.Ne, m.NeA, m.NhaloNode, **_arrays(m, Escale=esc))
+    assert ei.value.code == dgk.ERR_UNSUPPORTED
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.Solver.create("GENERAL", m.Nq, m.Ne, m.NeA, m.NhaloNode, **_arrays(m))
+    assert ei.value.code == dgk.ERR_UNSUPPORTED
+
+
+@pytest.mark.skipif(dgk.device_count() > 0, reason="host has a HIP device")
+def test_no_cpu_fallback_without_device():
+    m = oracle.Mesh(2, 2, 2, 2)
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.Solver.create("HIP", m.Nq, m.Ne, m.NeA, m.NhaloNode, **_arrays(m))
+    assert ei.value.code == dgk.ERR_DEVICE
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.Solver.structured("HIP", 4, 4, 4, 3)
+    assert ei.value.code == dgk.ERR_DEVICE
+
+
+def test_input_generator_matches_oracle_stream():
+    for seed in (1, 2, 77):
+        c = np.array([0, 1, 5, 1 << 20, (1 << 39) + 3], dtype=np.uint64)
+        ours = inputs.u01(seed, c)
+        ref = [oracle.u01(seed, int(x)) for x in c]
+        assert np.array_equal(ours, np.array(ref))
