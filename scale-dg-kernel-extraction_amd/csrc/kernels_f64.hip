@@ -251,7 +251,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
       for (int l = 0; l < NQ; ++l) {
         dx += Di[l] * (C::FX_T ? su[lb + j + l * NQ + k * NQ2] : su[lb + l + j * NQ + k * NQ2]);
         dy += Dj[l] * sv[lb + i + l * NQ + k * NQ2];
-        dz += D_[k + l * NQ] * fz[l];
+        dz += sD[k + l * NQ] * fz[l];  // wave-uniform LDS broadcast
       }
       const double lift = lmj * G[0 * NQ2 + i + k * NQ] + lpi * G[1 * NQ2 + j + k * NQ] +
                           lpj * G[2 * NQ2 + i + k * NQ] + lmi * G[3 * NQ2 + j + k * NQ] +
