@@ -1,0 +1,99 @@
+"""The mixed-precision device path ("HIP_MP": FP32 MFMA contractions and Lift
+with FP32 accumulation, FP64 state / flux jumps / RK) against the FP64
+references.
+
+Tolerance (stated by the workload definition): relative L2 error
+||q_mp - q_ref||_2 / ||q_ref||_2 <= 1e-5 after the time loop; a single
+tendency evaluation is held to the same bound relative to ||dqdt_ref||_2.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import dgk
+import oracle
+from dgk import inputs
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+def test_mp_tendency_every_order(p):
+    nx, ny, nz = (3, 2, 2) if p <= 6 else (2, 2, 2) if p <= 11 else (2, 1, 2)
+    m = oracle.Mesh(nx, ny, nz, p)
+    q, u, v, w = inputs.random_fields(nx, ny, nz, p, seed=300 + p)
+    ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    with dgk.Solver.create("HIP_MP", m.Nq, m.Ne, m.NeA, m.NhaloNode, m.D, m.Lift, m.VMapM,
+                           m.VMapP, m.halo_src_map, m.normal_fn, m.Escale, m.Fscale) as s:
+        s.set_fields(q, u, v, w)
+        got = s.cal_tend()
+    assert rel_l2(got, ref) <= TOL
+
+
+def test_mp_c1_run_against_oracle():
+    wl = inputs.c1_sine_noise()
+    m = oracle.Mesh(wl.NeX, wl.NeY, wl.NeZ, wl.p)
+    ref, _, _ = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, wl.nstep)
+    with dgk.Solver.structured("HIP_MP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        s.step(wl.dt, wl.nstep)
+        got = s.get_field(dgk.FIELD_Q)
+    assert rel_l2(got, ref) <= TOL
+
+
+def test_mp_c3_p15_against_reference_fixture():
+    """16^3 elements at p=15, 100 steps: against the reference's own run."""
+    z = np.load(GOLDEN / "c3_run.npz")
+    wl = inputs.c3_sine()
+    with dgk.Solver.structured("HIP_MP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        s.step(wl.dt, wl.nstep)
+        q = s.get_field(dgk.FIELD_Q).reshape(-1)
+    ref = z["general_sample"]
+    got = q[z["general_idx"]]
+    assert rel_l2(got, ref) <= TOL
+    assert abs(np.sqrt(np.sum(q * q)) - float(z["general_l2"])) <= TOL * float(z["general_l2"])
+
+
+def test_mp_c2_p7_against_fp64_device_path():
+    wl = inputs.c2_gaussian()
+    out = {}
+    for form in ("HIP", "HIP_MP"):
+        with dgk.Solver.structured(form, wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+            s.set_fields(wl.q, wl.u, wl.v, wl.w)
+            s.step(wl.dt, wl.nstep)
+            out[form] = s.get_field(dgk.FIELD_Q)
+    assert rel_l2(out["HIP_MP"], out["HIP"]) <= TOL
+
+
+def test_mp_rejects_non_separable_lift():
+    m = oracle.Mesh(2, 2, 2, 3)
+    L = m.Lift.copy()
+    L[0, 1, 2, 3] += 0.5
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.Solver.create("HIP_MP", m.Nq, m.Ne, m.NeA, m.NhaloNode, m.D, L, m.VMapM, m.VMapP,
+                          m.halo_src_map, m.normal_fn, m.Escale, m.Fscale)
+    assert ei.value.code == dgk.ERR_UNSUPPORTED
+
+
+def test_fp64_stored_lift_kernel_for_non_separable_lift():
+    """A Lift_mat that is not of tensor-product form runs through the
+    stored-Lift FP64 kernel and still matches the oracle."""
+    m = oracle.Mesh(2, 3, 2, 4)
+    L = m.Lift.copy()
+    L[2] += 0.01 * np.random.default_rng(3).standard_normal(L[2].shape)
+    m.Lift = np.ascontiguousarray(L)
+    q, u, v, w = inputs.random_fields(2, 3, 2, 4, seed=77)
+    ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    with dgk.Solver.create("HIP", m.Nq, m.Ne, m.NeA, m.NhaloNode, m.D, m.Lift, m.VMapM,
+                           m.VMapP, m.halo_src_map, m.normal_fn, m.Escale, m.Fscale) as s:
+        s.set_fields(q, u, v, w)
+        got = s.cal_tend()
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
