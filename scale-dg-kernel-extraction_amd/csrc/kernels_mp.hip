@@ -69,7 +69,10 @@ struct MP {
   static constexpr int OFF_G = OFF_W + 4 * FFIELD;          // [EPB][6][PPAD] floats
   static constexpr int OFF_D = OFF_G + 4 * EPB * 6 * PPAD;  // [NQ2] floats
   static constexpr int OFF_L = OFF_D + 4 * NQ2;             // [2*NQ] floats
-  static constexpr size_t LDS_BYTES = OFF_L + 4 * 2 * NQ;
+  static constexpr int OFF_Z = OFF_L + 4 * 2 * NQ;          // one 0.0f (padding operand)
+  static constexpr size_t LDS_BYTES = OFF_Z + 16;
+  static constexpr int JOBS = 3 * MTILES;                   // (direction, M tile) GEMM tiles
+  static constexpr int JPW = (JOBS + WAVES - 1) / WAVES;    // tiles per wavefront
 };
 
 // node of pencil r, position l along direction d (0 = x, 1 = y, 2 = z)
@@ -152,6 +155,7 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
   }
   for (int t = tid; t < NQ2; t += C::THREADS) sD[t] = D_[t];
   for (int t = tid; t < 2 * NQ; t += C::THREADS) sL[t] = ell_[t];
+  if (tid == 0) *(float *)(sm + C::OFF_Z) = 0.0f;
 
   const bool lane_ok = tid < EPB * NQ2;
   const int el = lane_ok ? tid / NQ2 : 0;
@@ -242,40 +246,64 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
   }
   __syncthreads();
 
-  // ---- 4. MFMA GEMMs, one (direction, 16-pencil tile) job at a time -------
+  // ---- 4. MFMA GEMMs over (direction, 16-pencil tile) jobs ---------------
+  // B = [E_d D^T ; l- ; l+] is the same for every direction up to E_d on its
+  // first NQ rows: lane (col = lane%16, k = lane/16) keeps its raw entries.
   {
-    const int row = lane & 15, kq = lane >> 4;  // A: row, k;  B: col = row, k = kq
-    for (int job = wave; job < 3 * C::MTILES; job += C::WAVES) {
+    const int row = lane & 15, kq = lane >> 4;
+    float braw[C::KSTEPS];
+    bool bscaled[C::KSTEPS];
+#pragma unroll
+    for (int ks = 0; ks < C::KSTEPS; ++ks) {
+      const int l = 4 * ks + kq;
+      braw[ks] = row >= NQ ? 0.f : l < NQ ? sD[row + l * NQ] : l == NQ ? sL[row]
+                                 : l == NQ + 1 ? sL[NQ + row] : 0.f;
+      bscaled[ks] = l < NQ;
+    }
+    const float *lds = (const float *)sm;
+    constexpr int ZERO = C::OFF_Z / 4;
+    f32x4 acc[C::JPW];
+#pragma unroll
+    for (int jj = 0; jj < C::JPW; ++jj) {
+      acc[jj] = f32x4{0.f, 0.f, 0.f, 0.f};
+      const int job = wave + jj * C::WAVES;
       const int d = job / C::MTILES, t = job - d * C::MTILES;
       const int tel = (t * 16) / PPAD, rb = (t * 16) - tel * PPAD;
-      const int64_t tslot = slot0 + tel;
-      if (tslot >= Ne_) continue;  // tile of an absent element (wave-uniform)
+      if (job >= C::JOBS || slot0 + tel >= Ne_) continue;  // wave-uniform
       const double *tg = geo_;
       if (geoid_) {
-        const int64_t te = order_ ? (int64_t)order_[tslot] : tslot;
-        tg = geo_ + (int64_t)geoid_[te] * kGeoStride;
+        const int64_t ts = slot0 + tel;
+        tg = geo_ + (int64_t)geoid_[order_ ? (int64_t)order_[ts] : ts] * kGeoStride;
       }
       const float Ed = (float)tg[kGeoEscale + d];
       const int fm = d == 0 ? 3 : d == 1 ? 0 : 4;  // -x, -y, -z faces
       const int fp = d == 0 ? 1 : d == 1 ? 2 : 5;  // +x, +y, +z faces
-      const float *F = sf[d] + tel * NP;
-      const float *Gm = sG + (tel * 6 + fm) * PPAD, *Gp = sG + (tel * 6 + fp) * PPAD;
+      const int fbase = (C::OFF_U / 4) + d * C::FFIELD + tel * NP;
+      const int gm = C::OFF_G / 4 + (tel * 6 + fm) * PPAD, gp = C::OFF_G / 4 + (tel * 6 + fp) * PPAD;
       const int r = rb + row;
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < C::KSTEPS; ++ks) {
         const int l = 4 * ks + kq;
-        float a = 0.f, b = 0.f;
-        if (r < NQ2) a = l < NQ ? F[pencil_node<NQ>(d, r, l)] : l == NQ ? Gm[r] : l == NQ + 1 ? Gp[r] : 0.f;
-        if (row < NQ) b = l < NQ ? Ed * sD[row + l * NQ] : l == NQ ? sL[row] : l == NQ + 1 ? sL[NQ + row] : 0.f;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+        int addr = ZERO;
+        if (r < NQ2) addr = l < NQ ? fbase + pencil_node<NQ>(d, r, l) : l == NQ ? gm + r
+                          : l == NQ + 1 ? gp + r : ZERO;
+        const float a = lds[addr];
+        const float b = bscaled[ks] ? Ed * braw[ks] : braw[ks];
+        acc[jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[jj], 0, 0, 0);
       }
-      // result rows rb + 4*kq + t4, column row: written back in place (this
-      // tile alone read those nodes of F_d)
+    }
+    // results (rows rb + 4*kq + t4, column row) go back in place: a tile
+    // alone read those nodes of F_d
+#pragma unroll
+    for (int jj = 0; jj < C::JPW; ++jj) {
+      const int job = wave + jj * C::WAVES;
+      const int d = job / C::MTILES, t = job - d * C::MTILES;
+      const int tel = (t * 16) / PPAD, rb = (t * 16) - tel * PPAD;
+      if (job >= C::JOBS || slot0 + tel >= Ne_) continue;
 #pragma unroll
       for (int t4 = 0; t4 < 4; ++t4) {
         const int ro = rb + 4 * kq + t4;
-        if (row < NQ && ro < NQ2) sf[d][tel * NP + pencil_node<NQ>(d, ro, row)] = acc[t4];
+        if (row < NQ && ro < NQ2) sf[d][tel * NP + pencil_node<NQ>(d, ro, row)] = acc[jj][t4];
       }
     }
   }
