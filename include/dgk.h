@@ -118,6 +118,13 @@ int dgk_solver_info(const dgk_solver *s, int *Nq, int64_t *Ne, int64_t *NeA,
 /* Device bytes held by the solver. */
 int dgk_device_bytes(const dgk_solver *s, int64_t *bytes);
 
+/* Device-side initial condition for a solver made by dgk_create_structured:
+ * q = exp(-r^2 / (2 sigma^2)) with r the periodic minimum-image distance of
+ * the node (set_element_position, mod_mesh.f90:332-353) to (cx,cy,cz), and
+ * constant velocity (ux,uy,uz); halos of u, v, w refreshed. */
+int dgk_init_gaussian(dgk_solver *s, double cx, double cy, double cz, double sigma,
+                      double ux, double uy, double uz);
+
 /* Upload a field, n = Np*Ne (owned part) or Np*NeA (the halo tail is
  * ignored: it is derived from the owned part).  Setting U, V or W refreshes
  * their halos once, as main.f90:160-162 does. */
@@ -165,6 +172,38 @@ int dgk_synchronize(dgk_solver *s);
 int dgk_set_kernel_timing(dgk_solver *s, int on);
 int dgk_timer_report(dgk_solver *s, double *seconds, int64_t *counts);
 int dgk_timer_reset(dgk_solver *s);
+
+/* ---- batched ensembles ----------------------------------------------------
+ * No reference counterpart: the reference runs one mesh per process.  A
+ * batch holds ninst independent periodic boxes (instance k: ne_xyz[3k..3k+2]
+ * elements, order p[k], domain dom_xyz[3k..3k+2] or the unit cube when NULL,
+ * time step dt[k]) resident in device memory.  Instances of one order are
+ * advanced together by the stage kernels (one launch per stage and order);
+ * dgk_batch_step captures one SSP-RK3 step of every order group in a
+ * hipGraph on first use and replays it nstep times.  Per-instance arithmetic
+ * is identical to a dgk_solver of that instance alone. */
+typedef struct dgk_batch dgk_batch;
+int dgk_batch_create(int evaltype, int ninst, const int *ne_xyz, const int *p,
+                     const double *dom_xyz, const double *dt,
+                     const char *operator_dir, dgk_batch **out);
+int dgk_batch_destroy(dgk_batch *b);
+/* order, elements and DOF of instance inst */
+int dgk_batch_info(const dgk_batch *b, int inst, int *p, int64_t *Ne, int64_t *dof);
+/* number of instances, order groups and device bytes */
+int dgk_batch_size(const dgk_batch *b, int *ninst, int *ngroups, int64_t *bytes);
+/* fields of one instance, n = Np*Ne of that instance (Q, U, V or W) */
+int dgk_batch_set_field(dgk_batch *b, int inst, int which, const double *f, int64_t n);
+int dgk_batch_get_field(dgk_batch *b, int inst, int which, double *f, int64_t n);
+/* device-side Gaussian-blob initial condition of one instance (as
+ * dgk_init_gaussian) */
+int dgk_batch_init_gaussian(dgk_batch *b, int inst, double cx, double cy, double cz,
+                            double sigma, double ux, double uy, double uz);
+/* graph replay (default) or eager launches */
+int dgk_batch_set_graph(dgk_batch *b, int on);
+int dgk_batch_step(dgk_batch *b, int nstep);
+int dgk_batch_synchronize(dgk_batch *b);
+/* device seconds and number of dgk_batch_step calls since the last read */
+int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,8 @@ struct dgk_solver {
   double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
   int32_t *order = nullptr;  // element processing order
   int axis_aligned = 0;
+  double *x1D = nullptr;     // LGL nodes (device), for device-side initial conditions
+  BoxShape box;              // structured box (NeX == 0 when built from arrays)
   int32_t *nbr = nullptr, *vmapP = nullptr, *halo = nullptr;
   double *geo = nullptr, *D = nullptr, *Lift = nullptr;
   uint32_t *geoid = nullptr;
@@ -181,20 +183,29 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
   a.flux_in = nullptr; a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D; a.Lift = s->Lift;
   a.ell = s->ell; a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.Ne = s->Ne;
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
+  a.dt_geo = 0;
   return a;
 }
 
+// One stage launch on `stream` (the solver's own stream unless a batch
+// supplies one); dt_geo takes the time step from each element's geometry
+// record instead of `dt`.
 int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out,
-                 int mode, double ca, double cb, double dt) {
+                 int mode, double ca, double cb, double dt, hipStream_t stream = nullptr,
+                 int dt_geo = 0) {
+  if (!stream) stream = s->stream;
   if (s->evaltype == DGK_EVAL_HIP_MP) {
     StageArgsMP a;
     a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
     a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
     a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.Ne = s->Ne;
     a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
-    return launch_stage_mp(s->Nq, a, s->stream);
+    a.dt_geo = dt_geo;
+    return launch_stage_mp(s->Nq, a, stream);
   }
-  return launch_stage_f64(s->Nq, stage_args(s, qin, q0, out, mode, ca, cb, dt), s->stream);
+  StageArgs a = stage_args(s, qin, q0, out, mode, ca, cb, dt);
+  a.dt_geo = dt_geo;
+  return launch_stage_f64(s->Nq, a, stream);
 }
 
 // Separable 1D form of the stored Lift: Lift(:,:,:,f) = I (x) l-/+ (x) I
@@ -303,6 +314,8 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
       return fail(set_error(DGK_ERR_DEVICE, "hipMemsetAsync failed"));
   if (hipStreamSynchronize(s->stream) != hipSuccess)
     return fail(set_error(DGK_ERR_DEVICE, "stream synchronize failed after upload"));
+  if ((rc = evaltype == DGK_EVAL_HIP_MP ? prepare_stage_mp(s->Nq) : prepare_stage_f64(s->Nq)))
+    return fail(rc);
   *out = s;
   return DGK_OK;
 }
@@ -365,7 +378,41 @@ int dgk_create_structured(int evaltype, int NeX, int NeY, int NeZ, int p,
   MeshTables t;
   DGK_TRY(build_tables_structured(NeX, NeY, NeZ, p, D.data(), L.data(), dom_x, dom_y,
                                   dom_z, t));
-  return create_from_tables(evaltype, t, out);
+  DGK_TRY(create_from_tables(evaltype, t, out));
+  dgk_solver *s = *out;
+  s->box.NeX = NeX; s->box.NeY = NeY; s->box.NeZ = NeZ;
+  s->box.dom[0] = dom_x; s->box.dom[1] = dom_y; s->box.dom[2] = dom_z;
+  int rc;
+  if ((rc = dalloc(s, &s->x1D, (int64_t)x.size())) || (rc = upload(s, s->x1D, x)) ||
+      (rc = (hipStreamSynchronize(s->stream) == hipSuccess ? DGK_OK
+                                                            : set_error(DGK_ERR_DEVICE, "sync")))) {
+    dgk_destroy(s);
+    *out = nullptr;
+    return rc;
+  }
+  return DGK_OK;
+}
+
+int dgk_init_gaussian(dgk_solver *s, double cx, double cy, double cz, double sigma,
+                      double ux, double uy, double uz) {
+  DGK_TRY(enter(s));
+  if (!s->x1D || s->box.NeX < 1)
+    return set_error(DGK_ERR_UNSUPPORTED, "device initial conditions need a structured solver");
+  if (!(sigma > 0)) return set_error(DGK_ERR_ARG, "sigma must be positive");
+  const double h[3] = {s->box.dom[0] / s->box.NeX, s->box.dom[1] / s->box.NeY,
+                       s->box.dom[2] / s->box.NeZ};
+  const double c[3] = {cx, cy, cz}, vel[3] = {ux, uy, uz};
+  DGK_TRY(launch_init_gaussian(s->q[s->cur], s->u, s->v, s->w, s->x1D, s->Nq, s->box.NeX,
+                               s->box.NeY, s->box.NeZ, h, c, sigma, vel, s->stream));
+  const int64_t owned = (int64_t)s->Np * s->Ne;
+  for (int which = DGK_FIELD_U; which <= DGK_FIELD_W; ++which) {
+    DGK_TRY(launch_halo_gather(field_ptr(s, which), s->halo, owned, s->NhaloNode, s->stream));
+    DGK_TRY(refresh_f32(s, which));
+  }
+  for (int f = 0; f < 4; ++f) s->set[f] = true;
+  s->q0_valid = s->flux_valid = s->dqdt_valid = false;
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  return DGK_OK;
 }
 
 int dgk_destroy(dgk_solver *s) {
@@ -609,6 +656,312 @@ int dgk_timer_reset(dgk_solver *s) {
     s->sec[i] = 0.0;
     s->cnt[i] = 0;
   }
+  return DGK_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched ensembles: instances of one polynomial order are concatenated into
+// one element space (build_tables_batch) and advanced by the same stage
+// kernels in one launch per stage; the per-instance time step comes from the
+// geometry record.  One SSP-RK3 step of every group is captured once into a
+// hipGraph (one parallel branch per order group) and replayed.
+// ---------------------------------------------------------------------------
+struct dgk_batch {
+  int evaltype = 0, device = 0;
+  hipStream_t stream = nullptr;  // capture origin
+  std::vector<dgk_solver *> groups;
+  std::vector<int> group_p;
+  struct Loc { int group; int64_t elem_off; int64_t Ne; int p; BoxShape box; };
+  std::vector<Loc> loc;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+  double step_seconds = 0.0;
+  int64_t step_calls = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+namespace {
+
+int batch_enter(const dgk_batch *b) {
+  clear_error();
+  if (!b) return set_error(DGK_ERR_ARG, "batch handle is NULL");
+  DGK_HIP(hipSetDevice(b->device));
+  return DGK_OK;
+}
+
+// The stages of one RK step of group g on `stream` (buffer rotation as in
+// dgk_step: A -> B -> C -> A).
+int batch_group_step(dgk_solver *s, hipStream_t stream) {
+  double *A = s->q[s->cur], *B = s->q[(s->cur + 1) % 3], *C = s->q[(s->cur + 2) % 3];
+  const double *in[3] = {A, B, C};
+  double *out[3] = {B, C, A};
+  for (int st = 0; st < 3; ++st)
+    DGK_TRY(launch_stage(s, in[st], A, out[st], 1, kRkA[st], kRkB[st], 0.0, stream, 1));
+  return DGK_OK;
+}
+
+int batch_enqueue_step(dgk_batch *b) {
+  DGK_HIP(hipEventRecord(b->fork, b->stream));
+  for (size_t g = 0; g < b->groups.size(); ++g) {
+    hipStream_t gs = b->groups[g]->stream;
+    DGK_HIP(hipStreamWaitEvent(gs, b->fork, 0));
+    DGK_TRY(batch_group_step(b->groups[g], gs));
+    DGK_HIP(hipEventRecord(b->join[g], gs));
+    DGK_HIP(hipStreamWaitEvent(b->stream, b->join[g], 0));
+  }
+  return DGK_OK;
+}
+
+int batch_capture(dgk_batch *b) {
+  DGK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_enqueue_step(b);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (rc != DGK_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return set_error(DGK_ERR_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));
+  b->graph = graph;
+  DGK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return DGK_OK;
+}
+
+int batch_loc(const dgk_batch *b, int inst, const dgk_batch::Loc **l) {
+  if (inst < 0 || inst >= (int)b->loc.size())
+    return set_error(DGK_ERR_ARG, "instance index " + std::to_string(inst) + " out of range");
+  *l = &b->loc[inst];
+  return DGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgk_batch_destroy(dgk_batch *b) {
+  clear_error();
+  if (!b) return DGK_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto &p : b->pending) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (dgk_solver *s : b->groups) dgk_destroy(s);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return DGK_OK;
+}
+
+int dgk_batch_create(int evaltype, int ninst, const int *ne_xyz, const int *p,
+                     const double *dom_xyz, const double *dt, const char *operator_dir,
+                     dgk_batch **out) {
+  clear_error();
+  if (!out) return set_error(DGK_ERR_ARG, "output handle pointer is NULL");
+  *out = nullptr;
+  if (ninst < 1 || !ne_xyz || !p || !dt) return set_error(DGK_ERR_ARG, "empty batch or NULL array");
+  if (evaltype != DGK_EVAL_HIP && evaltype != DGK_EVAL_HIP_MP)
+    return set_error(DGK_ERR_UNSUPPORTED, "batches run on HIP or HIP_MP");
+  auto *b = new dgk_batch();
+  auto fail = [&](int rc) {
+    std::string msg = dgk_last_error();
+    dgk_batch_destroy(b);
+    set_error(rc, msg);
+    return rc;
+  };
+  b->evaltype = evaltype;
+  if (hipGetDevice(&b->device) != hipSuccess) return fail(set_error(DGK_ERR_DEVICE, "hipGetDevice failed"));
+  b->loc.resize(ninst);
+  // group instances by order, in order of first appearance
+  std::vector<std::vector<int>> members;
+  for (int k = 0; k < ninst; ++k) {
+    int g = -1;
+    for (size_t j = 0; j < b->group_p.size(); ++j)
+      if (b->group_p[j] == p[k]) g = (int)j;
+    if (g < 0) {
+      g = (int)b->group_p.size();
+      b->group_p.push_back(p[k]);
+      members.emplace_back();
+    }
+    members[g].push_back(k);
+  }
+  for (size_t g = 0; g < b->group_p.size(); ++g) {
+    std::vector<double> x, D, L;
+    int rc = load_operators(operator_dir, b->group_p[g], x, D, L);
+    if (rc) return fail(rc);
+    std::vector<BatchInstance> inst;
+    for (int k : members[g]) {
+      BatchInstance bi;
+      bi.NeX = ne_xyz[3 * k]; bi.NeY = ne_xyz[3 * k + 1]; bi.NeZ = ne_xyz[3 * k + 2];
+      for (int d = 0; d < 3; ++d) bi.dom[d] = dom_xyz ? dom_xyz[3 * k + d] : 1.0;
+      bi.dt = dt[k];
+      inst.push_back(bi);
+    }
+    MeshTables t;
+    std::vector<int64_t> off;
+    if ((rc = build_tables_batch(b->group_p[g], D.data(), L.data(), inst, t, off))) return fail(rc);
+    dgk_solver *s = nullptr;
+    if ((rc = create_from_tables(evaltype, t, &s))) return fail(rc);
+    b->groups.push_back(s);
+    if ((rc = dalloc(s, &s->x1D, (int64_t)x.size())) || (rc = upload(s, s->x1D, x))) return fail(rc);
+    for (size_t j = 0; j < members[g].size(); ++j) {
+      const int k = members[g][j];
+      BoxShape bs;
+      bs.NeX = inst[j].NeX; bs.NeY = inst[j].NeY; bs.NeZ = inst[j].NeZ;
+      for (int d = 0; d < 3; ++d) bs.dom[d] = inst[j].dom[d];
+      b->loc[k] = {(int)g, off[j], off[j + 1] - off[j], b->group_p[g], bs};
+    }
+    s->set[DGK_FIELD_Q] = s->set[DGK_FIELD_U] = s->set[DGK_FIELD_V] = s->set[DGK_FIELD_W] = true;
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    return fail(set_error(DGK_ERR_DEVICE, "stream/event creation failed"));
+  b->join.resize(b->groups.size(), nullptr);
+  for (auto &e : b->join)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(set_error(DGK_ERR_DEVICE, "event creation failed"));
+  *out = b;
+  return DGK_OK;
+}
+
+int dgk_batch_info(const dgk_batch *b, int inst, int *p, int64_t *Ne, int64_t *dof) {
+  DGK_TRY(batch_enter(b));
+  const dgk_batch::Loc *l;
+  DGK_TRY(batch_loc(b, inst, &l));
+  const int64_t np = (int64_t)(l->p + 1) * (l->p + 1) * (l->p + 1);
+  if (p) *p = l->p;
+  if (Ne) *Ne = l->Ne;
+  if (dof) *dof = l->Ne * np;
+  return DGK_OK;
+}
+
+int dgk_batch_size(const dgk_batch *b, int *ninst, int *ngroups, int64_t *bytes) {
+  DGK_TRY(batch_enter(b));
+  if (ninst) *ninst = (int)b->loc.size();
+  if (ngroups) *ngroups = (int)b->groups.size();
+  if (bytes) {
+    int64_t t = 0;
+    for (const dgk_solver *s : b->groups) t += s->bytes;
+    *bytes = t;
+  }
+  return DGK_OK;
+}
+
+int dgk_batch_set_field(dgk_batch *b, int inst, int which, const double *f, int64_t n) {
+  DGK_TRY(batch_enter(b));
+  const dgk_batch::Loc *l;
+  DGK_TRY(batch_loc(b, inst, &l));
+  if (!f) return set_error(DGK_ERR_ARG, "field array is NULL");
+  if (which < DGK_FIELD_Q || which > DGK_FIELD_W) return set_error(DGK_ERR_ARG, "field id must be Q, U, V or W");
+  dgk_solver *s = b->groups[l->group];
+  const int64_t off = l->elem_off * s->Np, cnt = l->Ne * s->Np;
+  if (n != cnt) return set_error(DGK_ERR_ARG, "field length must be Np*Ne of the instance");
+  double *dst = field_ptr(s, which) + off;
+  DGK_HIP(hipMemcpyAsync(dst, f, sizeof(double) * cnt, hipMemcpyHostToDevice, s->stream));
+  if (which != DGK_FIELD_Q && s->evaltype == DGK_EVAL_HIP_MP) {
+    float *d32 = (which == DGK_FIELD_U ? s->u32 : which == DGK_FIELD_V ? s->v32 : s->w32) + off;
+    DGK_TRY(launch_to_f32(dst, d32, cnt, s->stream));
+  }
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_batch_get_field(dgk_batch *b, int inst, int which, double *f, int64_t n) {
+  DGK_TRY(batch_enter(b));
+  const dgk_batch::Loc *l;
+  DGK_TRY(batch_loc(b, inst, &l));
+  if (!f) return set_error(DGK_ERR_ARG, "field array is NULL");
+  if (which < DGK_FIELD_Q || which > DGK_FIELD_W) return set_error(DGK_ERR_ARG, "field id must be Q, U, V or W");
+  dgk_solver *s = b->groups[l->group];
+  const int64_t off = l->elem_off * s->Np, cnt = l->Ne * s->Np;
+  if (n != cnt) return set_error(DGK_ERR_ARG, "field length must be Np*Ne of the instance");
+  DGK_HIP(hipStreamSynchronize(b->stream));
+  DGK_HIP(hipMemcpyAsync(f, field_ptr(s, which) + off, sizeof(double) * cnt,
+                         hipMemcpyDeviceToHost, s->stream));
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_batch_init_gaussian(dgk_batch *b, int inst, double cx, double cy, double cz,
+                            double sigma, double ux, double uy, double uz) {
+  DGK_TRY(batch_enter(b));
+  const dgk_batch::Loc *l;
+  DGK_TRY(batch_loc(b, inst, &l));
+  if (!(sigma > 0)) return set_error(DGK_ERR_ARG, "sigma must be positive");
+  dgk_solver *s = b->groups[l->group];
+  const int64_t off = l->elem_off * s->Np, cnt = l->Ne * s->Np;
+  const double h[3] = {l->box.dom[0] / l->box.NeX, l->box.dom[1] / l->box.NeY,
+                       l->box.dom[2] / l->box.NeZ};
+  const double c[3] = {cx, cy, cz}, vel[3] = {ux, uy, uz};
+  DGK_HIP(hipStreamSynchronize(b->stream));
+  DGK_TRY(launch_init_gaussian(s->q[s->cur] + off, s->u + off, s->v + off, s->w + off, s->x1D,
+                               s->Nq, l->box.NeX, l->box.NeY, l->box.NeZ, h, c, sigma, vel,
+                               s->stream));
+  if (s->evaltype == DGK_EVAL_HIP_MP) {
+    DGK_TRY(launch_to_f32(s->u + off, s->u32 + off, cnt, s->stream));
+    DGK_TRY(launch_to_f32(s->v + off, s->v32 + off, cnt, s->stream));
+    DGK_TRY(launch_to_f32(s->w + off, s->w32 + off, cnt, s->stream));
+  }
+  DGK_HIP(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_batch_set_graph(dgk_batch *b, int on) {
+  DGK_TRY(batch_enter(b));
+  b->use_graph = on != 0;
+  return DGK_OK;
+}
+
+int dgk_batch_step(dgk_batch *b, int nstep) {
+  DGK_TRY(batch_enter(b));
+  if (nstep < 0) return set_error(DGK_ERR_ARG, "nstep must be >= 0");
+  if (nstep == 0) return DGK_OK;
+  // fields uploaded on the group streams are ordered before the step
+  for (dgk_solver *s : b->groups) DGK_HIP(hipStreamSynchronize(s->stream));
+  if (b->use_graph && !b->exec) DGK_TRY(batch_capture(b));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  DGK_HIP(hipEventCreate(&t0));
+  DGK_HIP(hipEventCreate(&t1));
+  DGK_HIP(hipEventRecord(t0, b->stream));
+  for (int it = 0; it < nstep; ++it) {
+    if (b->use_graph) DGK_HIP(hipGraphLaunch(b->exec, b->stream));
+    else DGK_TRY(batch_enqueue_step(b));
+  }
+  DGK_HIP(hipEventRecord(t1, b->stream));
+  b->pending.push_back({t0, t1});
+  return DGK_OK;
+}
+
+int dgk_batch_synchronize(dgk_batch *b) {
+  DGK_TRY(batch_enter(b));
+  DGK_HIP(hipStreamSynchronize(b->stream));
+  for (auto &p : b->pending) {
+    float ms = 0.f;
+    DGK_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    b->step_seconds += 1e-3 * ms;
+    b->step_calls += 1;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  b->pending.clear();
+  return DGK_OK;
+}
+
+int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls) {
+  DGK_TRY(dgk_batch_synchronize(b));
+  if (seconds) *seconds = b->step_seconds;
+  if (calls) *calls = b->step_calls;
+  b->step_seconds = 0.0;
+  b->step_calls = 0;
   return DGK_OK;
 }
 

@@ -42,6 +42,7 @@ constexpr int kGeoStride = 32;  // doubles per record (27 used)
 constexpr int kGeoEscale = 0;
 constexpr int kGeoFscale = 3;
 constexpr int kGeoNormal = 9;  // + 3*f + d
+constexpr int kGeoDt = 27;     // time step of the element's instance (batched solves)
 
 // Node index (0-based) inside an element of face node (a,b) of face f on the
 // element's own side, in the reference's Fmask order (mod_mesh.f90:278-324):
@@ -110,6 +111,33 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
 int load_operators(const char *dir, int p, std::vector<double> &x1D,
                    std::vector<double> &D, std::vector<double> &Lift);
 
+// One instance of a batch: a periodic box of NeX x NeY x NeZ elements.
+struct BatchInstance {
+  int NeX, NeY, NeZ;
+  double dom[3];
+  double dt;
+};
+
+// Shape of a structured periodic box (kept by solvers built from one, for
+// device-side initial conditions).
+struct BoxShape {
+  int NeX = 0, NeY = 0, NeZ = 0;
+  double dom[3] = {0, 0, 0};
+};
+// Tables of several same-order instances concatenated into one element
+// space: element indices (and neighbour entries) of instance k are offset by
+// the elements of instances 0..k-1; each instance gets its own geometry
+// record, carrying its time step at kGeoDt.  No halo (the neighbour table
+// addresses the periodic partner directly).
+int build_tables_batch(int p, const double *D1D, const double *Lift_mat,
+                       const std::vector<BatchInstance> &inst, MeshTables &t,
+                       std::vector<int64_t> &elem_offset);
+
+// Function attributes (dynamic LDS limits) of the stage kernels of order Nq,
+// set outside any stream capture.
+int prepare_stage_f64(int Nq);
+int prepare_stage_mp(int Nq);
+
 // ---------------------------------------------------------------------------
 // Kernel parameter blocks and launchers (kernels_f64.hip, kernels_mp.hip).
 // ---------------------------------------------------------------------------
@@ -131,6 +159,7 @@ struct StageArgs {
   double ca, cb, dt;      // RK stage: out = ca*q0 + cb*(q + dt*tend)
   int mode;               // 0 = tendency, 1 = RK stage
   int axis_aligned;       // face normals along the face axes (see MeshTables)
+  int dt_geo;             // RK time step from the geometry record (per instance)
 };
 
 // Mixed-precision operands (FP32 copies of the constant inputs).
@@ -150,6 +179,7 @@ struct StageArgsMP {
   double ca, cb, dt;
   int mode;
   int axis_aligned;
+  int dt_geo;
 };
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);
@@ -166,6 +196,10 @@ int launch_rk_update(double *q, const double *q0, const double *dqdt,
 int launch_minmax(const double *q, int64_t n, double *partial, int nblk,
                   double *result, hipStream_t s);
 int launch_to_f32(const double *src, float *dst, int64_t n, hipStream_t s);
+int launch_init_gaussian(double *q, double *u, double *v, double *w, const double *x1D,
+                         int Nq, int NeX, int NeY, int NeZ, const double h[3],
+                         const double c[3], double sigma, const double vel[3],
+                         hipStream_t s);
 constexpr int kMinmaxBlocks = 1024;
 
 }  // namespace dgk

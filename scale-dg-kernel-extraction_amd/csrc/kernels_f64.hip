@@ -83,7 +83,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
           const double *__restrict__ ell_, const double *__restrict__ geo_,
           const uint32_t *__restrict__ geoid_, const int32_t *__restrict__ order_,
           const int64_t Ne_, const double ca_, const double cb_, const double dt_,
-          const int mode_, const int axis_aligned_) {
+          const int mode_, const int axis_aligned_, const int dt_geo_) {
   // Read-only operands are __restrict__ kernel parameters, so wave-uniform
   // reads of them (operators, geometry, neighbour table) are scalar loads.
   using C = Staged<NQ>;
@@ -261,7 +261,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
       if (!rk) {
         out_[idx] = tend;
       } else {
-        const double upd = cb_ * (qc[k] + dt_ * tend);
+        const double upd = cb_ * (qc[k] + (dt_geo_ ? g[kGeoDt] : dt_) * tend);
         out_[idx] = use_q0 ? ca_ * q0v[k] + upd : upd;
       }
     }
@@ -396,11 +396,25 @@ stage_f64_lift(const StageArgs P) {
       if (!rk) {
         P.out[idx] = tend;
       } else {
-        const double upd = P.cb * (q[k] + P.dt * tend);
+        const double upd = P.cb * (q[k] + (P.dt_geo ? g[kGeoDt] : P.dt) * tend);
         P.out[idx] = use_q0 ? P.ca * P.q0[idx] + upd : upd;
       }
     }
   }
+}
+
+template <int NQ>
+int prepare_nq() {
+  static int prepared_dev = -1;
+  int dev = 0;
+  DGK_HIP(hipGetDevice(&dev));
+  if (prepared_dev != dev) {
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)Staged<NQ>::LDS_BYTES));
+    prepared_dev = dev;
+  }
+  return DGK_OK;
 }
 
 template <int NQ>
@@ -409,15 +423,10 @@ int launch_nq(const StageArgs &a, hipStream_t s) {
     using T = Staged<NQ>;
     const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
     if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-    static bool attr_set = false;
-    if (!attr_set) {
-      DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)T::LDS_BYTES));
-      attr_set = true;
-    }
+    DGK_TRY(prepare_nq<NQ>());
     hipLaunchKernelGGL(stage_f64<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
                        a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
-                       a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned);
+                       a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo);
   } else {
     using T = Tile<NQ>;
     const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
@@ -521,6 +530,34 @@ __global__ void minmax_final(const double *partial, int nblk, double *result) {
   }
 }
 
+// Periodic Gaussian blob exp(-r^2/(2 sigma^2)) (minimum-image distance) and
+// constant velocity on a NeX x NeY x NeZ box, node coordinates as in
+// set_element_position (mod_mesh.f90:332-353).
+__global__ void init_gaussian(double *q, double *u, double *v, double *w,
+                              const double *__restrict__ x1D, int Nq, int NeX, int NeY,
+                              double hx, double hy, double hz, double cx, double cy,
+                              double cz, double sigma, double ux, double uy, double uz,
+                              int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int Np = Nq * Nq * Nq;
+  const int64_t e = t / Np;
+  const int nd = (int)(t - e * Np);
+  const int64_t ie = e % NeX, je = (e / NeX) % NeY, ke = e / ((int64_t)NeX * NeY);
+  const int a = nd % Nq, b = (nd / Nq) % Nq, c = nd / (Nq * Nq);
+  const double x = (((double)(ie + 1)) - 0.5) * hx + (0.5 * hx) * x1D[a];
+  const double y = (((double)(je + 1)) - 0.5) * hy + (0.5 * hy) * x1D[b];
+  const double z = (((double)(ke + 1)) - 0.5) * hz + (0.5 * hz) * x1D[c];
+  double r2 = 0.0, d;
+  d = x - cx; d = d - rint(d); r2 += d * d;
+  d = y - cy; d = d - rint(d); r2 += d * d;
+  d = z - cz; d = d - rint(d); r2 += d * d;
+  q[t] = exp(-r2 / (2.0 * sigma * sigma));
+  u[t] = ux;
+  v[t] = uy;
+  w[t] = uz;
+}
+
 __global__ void to_f32(const double *__restrict__ src, float *__restrict__ dst,
                        int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -551,6 +588,27 @@ int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   case 14: return launch_nq<14>(a, s);
   case 15: return launch_nq<15>(a, s);
   case 16: return launch_nq<16>(a, s);
+  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  }
+}
+
+int prepare_stage_f64(int Nq) {
+  switch (Nq) {
+  case 2: return prepare_nq<2>();
+  case 3: return prepare_nq<3>();
+  case 4: return prepare_nq<4>();
+  case 5: return prepare_nq<5>();
+  case 6: return prepare_nq<6>();
+  case 7: return prepare_nq<7>();
+  case 8: return prepare_nq<8>();
+  case 9: return prepare_nq<9>();
+  case 10: return prepare_nq<10>();
+  case 11: return prepare_nq<11>();
+  case 12: return prepare_nq<12>();
+  case 13: return prepare_nq<13>();
+  case 14: return prepare_nq<14>();
+  case 15: return prepare_nq<15>();
+  case 16: return prepare_nq<16>();
   default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
   }
 }
@@ -590,6 +648,19 @@ int launch_minmax(const double *q, int64_t n, double *partial, int nblk,
   hipLaunchKernelGGL(minmax_partial, dim3(nblk), dim3(256), 0, s, q, n, partial);
   DGK_HIP(hipGetLastError());
   hipLaunchKernelGGL(minmax_final, dim3(1), dim3(256), 0, s, partial, nblk, result);
+  DGK_HIP(hipGetLastError());
+  return DGK_OK;
+}
+
+int launch_init_gaussian(double *q, double *u, double *v, double *w, const double *x1D,
+                         int Nq, int NeX, int NeY, int NeZ, const double h[3],
+                         const double c[3], double sigma, const double vel[3],
+                         hipStream_t s) {
+  const int64_t n = (int64_t)NeX * NeY * NeZ * Nq * Nq * Nq;
+  if (n == 0) return DGK_OK;
+  hipLaunchKernelGGL(init_gaussian, dim3(grid1d(n, 256)), dim3(256), 0, s, q, u, v, w, x1D, Nq,
+                     NeX, NeY, h[0], h[1], h[2], c[0], c[1], c[2], sigma, vel[0], vel[1],
+                     vel[2], n);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }

@@ -90,7 +90,7 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
          const float *__restrict__ D_, const float *__restrict__ ell_,
          const double *__restrict__ geo_, const uint32_t *__restrict__ geoid_,
          const int32_t *__restrict__ order_, const int64_t Ne_, const double ca_,
-         const double cb_, const double dt_, const int mode_, const int axis_aligned_) {
+         const double cb_, const double dt_, const int mode_, const int axis_aligned_, const int dt_geo_) {
   using C = MP<NQ>;
   constexpr int NQ2 = C::NQ2, NP = C::NP, EPB = C::EPB, NFT = 6 * NQ2, PPAD = C::PPAD;
   extern __shared__ double smem_mp[];
@@ -319,7 +319,7 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
       if (!rk) {
         out_[idx] = tend;
       } else {
-        const double upd = cb_ * (qc[k] + dt_ * tend);
+        const double upd = cb_ * (qc[k] + (dt_geo_ ? g[kGeoDt] : dt_) * tend);
         out_[idx] = use_q0 ? ca_ * q0v[k] + upd : upd;
       }
     }
@@ -327,24 +327,54 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
 }
 
 template <int NQ>
+int prepare_mp_nq() {
+  static int prepared_dev = -1;
+  int dev = 0;
+  DGK_HIP(hipGetDevice(&dev));
+  if (prepared_dev != dev) {
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_mp<NQ>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)MP<NQ>::LDS_BYTES));
+    prepared_dev = dev;
+  }
+  return DGK_OK;
+}
+
+template <int NQ>
 int launch_mp_nq(const StageArgsMP &a, hipStream_t s) {
   using T = MP<NQ>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  static bool attr_set = false;
-  if (!attr_set) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_mp<NQ>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)T::LDS_BYTES));
-    attr_set = true;
-  }
+  DGK_TRY(prepare_mp_nq<NQ>());
   hipLaunchKernelGGL(stage_mp<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
                      a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr, a.vmapP, a.D, a.ell, a.geo,
-                     a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned);
+                     a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }
 
 }  // namespace
+
+int prepare_stage_mp(int Nq) {
+  switch (Nq) {
+  case 2: return prepare_mp_nq<2>();
+  case 3: return prepare_mp_nq<3>();
+  case 4: return prepare_mp_nq<4>();
+  case 5: return prepare_mp_nq<5>();
+  case 6: return prepare_mp_nq<6>();
+  case 7: return prepare_mp_nq<7>();
+  case 8: return prepare_mp_nq<8>();
+  case 9: return prepare_mp_nq<9>();
+  case 10: return prepare_mp_nq<10>();
+  case 11: return prepare_mp_nq<11>();
+  case 12: return prepare_mp_nq<12>();
+  case 13: return prepare_mp_nq<13>();
+  case 14: return prepare_mp_nq<14>();
+  case 15: return prepare_mp_nq<15>();
+  case 16: return prepare_mp_nq<16>();
+  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  }
+}
 
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;

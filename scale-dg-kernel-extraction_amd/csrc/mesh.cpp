@@ -179,6 +179,58 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
   return DGK_OK;
 }
 
+int build_tables_batch(int p, const double *D1D, const double *Lift_mat,
+                       const std::vector<BatchInstance> &inst, MeshTables &t,
+                       std::vector<int64_t> &elem_offset) {
+  if (p < kMinNq - 1 || p > kMaxNq - 1)
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+  if (inst.empty()) return set_error(DGK_ERR_ARG, "empty batch");
+  const int Nq = p + 1;
+  t.Nq = Nq; t.Np = Nq * Nq * Nq; t.Nfp = Nq * Nq; t.NfpTot = 6 * t.Nfp;
+  int64_t total = 0;
+  elem_offset.assign(inst.size() + 1, 0);
+  for (size_t k = 0; k < inst.size(); ++k) {
+    const BatchInstance &b = inst[k];
+    if (b.NeX < 1 || b.NeY < 1 || b.NeZ < 1)
+      return set_error(DGK_ERR_ARG, "instance " + std::to_string(k) + ": element counts must be >= 1");
+    if (!(b.dom[0] > 0 && b.dom[1] > 0 && b.dom[2] > 0) || !(b.dt > 0))
+      return set_error(DGK_ERR_ARG, "instance " + std::to_string(k) + ": domain and dt must be positive");
+    elem_offset[k] = total;
+    total += (int64_t)b.NeX * b.NeY * b.NeZ;
+  }
+  elem_offset[inst.size()] = total;
+  if ((int64_t)t.Np * total >= (int64_t)INT32_MAX)
+    return set_error(DGK_ERR_UNSUPPORTED, "batch exceeds 2^31 nodes (int32 maps)");
+  t.Ne = total; t.NeA = total; t.NhaloNode = 0;
+  t.D.assign(D1D, D1D + (size_t)Nq * Nq);
+  t.Lift.assign(Lift_mat, Lift_mat + (size_t)t.Np * 6);
+  t.nbr.resize((size_t)total * 6);
+  t.vmapP.clear();
+  t.halo_src.clear();
+  t.geo.assign(inst.size() * kGeoStride, 0.0);
+  t.geoid.resize(total);
+  for (size_t k = 0; k < inst.size(); ++k) {
+    const BatchInstance &b = inst[k];
+    double *rec = &t.geo[k * kGeoStride];
+    fill_box_geo(b.dom[0] / b.NeX, b.dom[1] / b.NeY, b.dom[2] / b.NeZ, rec);
+    rec[kGeoDt] = b.dt;
+    const int64_t off = elem_offset[k];
+    for (int kk = 0; kk < b.NeZ; ++kk)
+      for (int j = 0; j < b.NeY; ++j)
+        for (int i = 0; i < b.NeX; ++i) {
+          const int64_t e = i + (int64_t)j * b.NeX + (int64_t)kk * b.NeX * b.NeY;
+          for (int f = 0; f < 6; ++f) {
+            bool bnd;
+            t.nbr[(off + e) * 6 + f] = (int32_t)(off + neighbour(i, j, kk, f, b.NeX, b.NeY, b.NeZ, bnd));
+          }
+          t.geoid[off + e] = (uint32_t)k;
+        }
+  }
+  t.order.clear();
+  t.axis_aligned = true;
+  return DGK_OK;
+}
+
 int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
                                 int64_t NhaloNode, const double *D1D,
                                 const double *Lift_mat, const int *VMapM,

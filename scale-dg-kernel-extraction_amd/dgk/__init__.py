@@ -61,6 +61,18 @@ _SIGNATURES = {
     "dgk_set_kernel_timing": (_I, [_P, _I]),
     "dgk_timer_report": (_I, [_P, _P, _P]),
     "dgk_timer_reset": (_I, [_P]),
+    "dgk_init_gaussian": (_I, [_P, _D, _D, _D, _D, _D, _D, _D]),
+    "dgk_batch_create": (_I, [_I, _I, _P, _P, _P, _P, _S, _P]),
+    "dgk_batch_destroy": (_I, [_P]),
+    "dgk_batch_info": (_I, [_P, _I, _P, _P, _P]),
+    "dgk_batch_size": (_I, [_P, _P, _P, _P]),
+    "dgk_batch_set_field": (_I, [_P, _I, _I, _P, _L]),
+    "dgk_batch_get_field": (_I, [_P, _I, _I, _P, _L]),
+    "dgk_batch_init_gaussian": (_I, [_P, _I, _D, _D, _D, _D, _D, _D, _D]),
+    "dgk_batch_set_graph": (_I, [_P, _I]),
+    "dgk_batch_step": (_I, [_P, _I]),
+    "dgk_batch_synchronize": (_I, [_P]),
+    "dgk_batch_timer": (_I, [_P, _P, _P]),
 }
 
 _lib = None
@@ -289,3 +301,85 @@ class Solver:
 
     def timer_reset(self) -> None:
         _check(lib().dgk_timer_reset(self._h))
+
+    def init_gaussian(self, centre, sigma, vel) -> None:
+        """Device-side Gaussian blob and constant velocity (structured solvers)."""
+        _check(lib().dgk_init_gaussian(self._h, *(float(c) for c in centre), float(sigma),
+                                       *(float(c) for c in vel)))
+
+
+class Batch:
+    """Independent periodic boxes advanced together (dgk_batch_*).
+
+    instances: sequence of dicts with keys ne (NeX, NeY, NeZ), p, dt and
+    optionally dom (Lx, Ly, Lz)."""
+
+    def __init__(self, evaltype, instances, operator_dir=OPERATOR_DIR):
+        if isinstance(evaltype, str):
+            evaltype = evaltype_id(evaltype)
+        n = len(instances)
+        ne = np.array([list(it["ne"]) for it in instances], np.int32).reshape(-1)
+        p = np.array([it["p"] for it in instances], np.int32)
+        dom = np.array([list(it.get("dom", (1.0, 1.0, 1.0))) for it in instances]).reshape(-1)
+        dt = np.array([it["dt"] for it in instances], np.float64)
+        h = C.c_void_p()
+        _check(lib().dgk_batch_create(evaltype, n, _ptr(ne), _ptr(p), _ptr(dom), _ptr(dt),
+                                      str(operator_dir).encode(), C.byref(h)))
+        self._h = h
+        self.n = n
+
+    def close(self) -> None:
+        if self._h:
+            _check(lib().dgk_batch_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self, inst: int) -> dict:
+        p, ne, dof = C.c_int(0), C.c_int64(0), C.c_int64(0)
+        _check(lib().dgk_batch_info(self._h, inst, C.byref(p), C.byref(ne), C.byref(dof)))
+        return dict(p=p.value, Ne=ne.value, dof=dof.value)
+
+    def size(self) -> dict:
+        n, g, b = C.c_int(0), C.c_int(0), C.c_int64(0)
+        _check(lib().dgk_batch_size(self._h, C.byref(n), C.byref(g), C.byref(b)))
+        return dict(instances=n.value, groups=g.value, device_bytes=b.value)
+
+    def set_field(self, inst: int, which: int, f) -> None:
+        a = _f64(f).reshape(-1)
+        _check(lib().dgk_batch_set_field(self._h, inst, which, _ptr(a), a.size))
+
+    def get_field(self, inst: int, which: int = FIELD_Q) -> np.ndarray:
+        info = self.info(inst)
+        Np = (info["p"] + 1) ** 3
+        out = np.zeros((info["Ne"], Np))
+        _check(lib().dgk_batch_get_field(self._h, inst, which, _ptr(out), out.size))
+        return out
+
+    def init_gaussian(self, inst: int, centre, sigma, vel) -> None:
+        _check(lib().dgk_batch_init_gaussian(self._h, inst, *(float(c) for c in centre),
+                                             float(sigma), *(float(c) for c in vel)))
+
+    def set_graph(self, on: bool) -> None:
+        _check(lib().dgk_batch_set_graph(self._h, int(bool(on))))
+
+    def step(self, nstep: int) -> None:
+        _check(lib().dgk_batch_step(self._h, int(nstep)))
+
+    def synchronize(self) -> None:
+        _check(lib().dgk_batch_synchronize(self._h))
+
+    def timer(self):
+        sec, calls = C.c_double(0), C.c_int64(0)
+        _check(lib().dgk_batch_timer(self._h, C.byref(sec), C.byref(calls)))
+        return sec.value, calls.value

@@ -154,3 +154,33 @@ def random_fields(NeX, NeY, NeZ, p, seed, vel_scale=1.0):
     v = vel_scale * (2.0 * u01(seed, c + np.uint64(2 * n)) - 1.0)
     w = vel_scale * (2.0 * u01(seed, c + np.uint64(3 * n)) - 1.0)
     return (q.reshape(Ne, Np), u.reshape(Ne, Np), v.reshape(Ne, Np), w.reshape(Ne, Np))
+
+
+def ensemble(n=512, seed=4, ne_choices=(4, 6, 8, 12, 16), p_choices=(3, 5, 7, 11),
+             sigma_range=(0.05, 0.2), nstep=50, cfl=0.1):
+    """Batched-ensemble instances: instance k draws, from counters of seed
+    `seed` starting at PARAM_BASE + 16k, Ne per axis and p uniformly from the
+    given choices, a velocity with components U[-1,1], a Gaussian-blob centre
+    U[0,1)^3 and sigma U[sigma_range]; dt from CFL `cfl`."""
+    out = []
+    for k in range(n):
+        r = u01(seed, PARAM_BASE + 16 * k + np.arange(9, dtype=np.uint64))
+        ne = int(ne_choices[min(int(r[0] * len(ne_choices)), len(ne_choices) - 1)])
+        p = int(p_choices[min(int(r[1] * len(p_choices)), len(p_choices) - 1)])
+        vel = tuple(float(2.0 * r[2 + c] - 1.0) for c in range(3))
+        centre = tuple(float(r[5 + c]) for c in range(3))
+        sigma = float(sigma_range[0] + (sigma_range[1] - sigma_range[0]) * r[8])
+        x1D = lgl_nodes(p)
+        out.append(dict(ne=(ne, ne, ne), p=p, vel=vel, centre=centre, sigma=sigma,
+                        nstep=nstep, dt=cfl_dt(x1D, (ne, ne, ne), vel, cfl=cfl)))
+    return out
+
+
+_LGL = {}
+
+
+def lgl_nodes(p):
+    """x1D of operator_data/p{p}.dat (cached)."""
+    if p not in _LGL:
+        _LGL[p] = load_operators(p)[0]
+    return _LGL[p]
