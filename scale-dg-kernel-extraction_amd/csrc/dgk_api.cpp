@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "dgk_internal.hpp"
+#include "solver.hpp"
 
 namespace dgk {
 
@@ -21,78 +21,13 @@ int set_error(int code, const std::string &msg) {
 }
 void clear_error() { g_last_error.clear(); }
 
-// SSP-RK3 Shu-Osher coefficients (mod_common.f90:28-30).
-constexpr double kRkA[3] = {0.0, 0.75, 1.0 / 3.0};
-constexpr double kRkB[3] = {1.0, 0.25, 2.0 / 3.0};
-
-// Timer slots (names of main.f90:175-177 and mod_advect3d_eq.f90:34-35).
-enum { T_MAIN = 0, T_CALTEND = 1, T_FLUX = 2, T_DQDT = 3, T_STAGE = 4, T_N = 5 };
 
 }  // namespace dgk
 
 using namespace dgk;
 
-struct dgk_solver {
-  int evaltype = 0;
-  int device = 0;
-  int Nq = 0, Np = 0, Nfp = 0, NfpTot = 0;
-  int64_t Ne = 0, NeA = 0, NhaloNode = 0;
-  int64_t nfield = 0;  // Np*Ne + NhaloNode: owned part + halo tail
-  hipStream_t stream = nullptr;
-  double *q[3] = {nullptr, nullptr, nullptr};
-  int cur = 0;
-  double *u = nullptr, *v = nullptr, *w = nullptr;
-  double *q0 = nullptr, *dqdt = nullptr, *flux = nullptr;
-  bool flux_valid = false, dqdt_valid = false, q0_valid = false;
-  float *u32 = nullptr, *v32 = nullptr, *w32 = nullptr;
-  float *D32 = nullptr, *ell32 = nullptr;
-  double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
-  int32_t *order = nullptr;  // element processing order
-  int axis_aligned = 0;
-  double *x1D = nullptr;     // LGL nodes (device), for device-side initial conditions
-  BoxShape box;              // structured box (NeX == 0 when built from arrays)
-  int32_t *nbr = nullptr, *vmapP = nullptr, *halo = nullptr;
-  double *geo = nullptr, *D = nullptr, *Lift = nullptr;
-  uint32_t *geoid = nullptr;
-  double *mm = nullptr;  // min/max partials + result
-  bool set[4] = {false, false, false, false};
-  int64_t bytes = 0;
-  std::vector<void *> allocs;
-  // timers
-  bool kernel_timing = false;
-  double sec[T_N] = {0, 0, 0, 0, 0};
-  int64_t cnt[T_N] = {0, 0, 0, 0, 0};
-  struct Pending { int slot; hipEvent_t a, b; };
-  std::vector<Pending> pending;
-  std::vector<hipEvent_t> free_events;
-};
 
 namespace {
-
-template <typename T>
-int dalloc(dgk_solver *s, T **p, int64_t n) {
-  *p = nullptr;
-  if (n <= 0) return DGK_OK;
-  void *ptr = nullptr;
-  hipError_t e = hipMalloc(&ptr, sizeof(T) * (size_t)n);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return set_error(DGK_ERR_NOMEM, "hipMalloc of " + std::to_string(sizeof(T) * n) +
-                                        " bytes failed: " + hipGetErrorString(e));
-  }
-  s->allocs.push_back(ptr);
-  s->bytes += (int64_t)sizeof(T) * n;
-  *p = (T *)ptr;
-  return DGK_OK;
-}
-
-template <typename T>
-int upload(dgk_solver *s, T *dst, const std::vector<T> &src) {
-  if (src.empty()) return DGK_OK;
-  DGK_HIP(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice,
-                         s->stream));
-  return DGK_OK;
-}
 
 hipEvent_t take_event(dgk_solver *s) {
   if (!s->free_events.empty()) {
@@ -159,6 +94,10 @@ int require_fields(const dgk_solver *s) {
   return DGK_OK;
 }
 
+}  // namespace
+
+namespace dgk {
+
 double *field_ptr(dgk_solver *s, int which) {
   switch (which) {
   case DGK_FIELD_Q: return s->q[s->cur];
@@ -191,8 +130,7 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
 // supplies one); dt_geo takes the time step from each element's geometry
 // record instead of `dt`.
 int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out,
-                 int mode, double ca, double cb, double dt, hipStream_t stream = nullptr,
-                 int dt_geo = 0) {
+                 int mode, double ca, double cb, double dt, hipStream_t stream, int dt_geo) {
   if (!stream) stream = s->stream;
   if (s->evaltype == DGK_EVAL_HIP_MP) {
     StageArgsMP a;
@@ -207,6 +145,10 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
   a.dt_geo = dt_geo;
   return launch_stage_f64(s->Nq, a, stream);
 }
+
+}  // namespace dgk
+
+namespace {
 
 // Separable 1D form of the stored Lift: Lift(:,:,:,f) = I (x) l-/+ (x) I
 // along the face normal (faces -y,+x,+y,-x,-z,+z use l-(j), l+(i), l+(j),
@@ -240,6 +182,10 @@ double separable_lift(const MeshTables &t, std::vector<double> &ell) {
 // applies the 1D form, otherwise it reads Lift_mat as stored.
 constexpr double kSeparableF64 = 1e-12;
 constexpr double kSeparableMP = 1e-10;
+
+}  // namespace
+
+namespace dgk {
 
 int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   if (!out) return set_error(DGK_ERR_ARG, "output handle pointer is NULL");
@@ -320,7 +266,7 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   return DGK_OK;
 }
 
-}  // namespace
+}  // namespace dgk
 
 extern "C" {
 
