@@ -97,9 +97,11 @@ int dgk_mesh_build(int NeX, int NeY, int NeZ, int p, const double *x1D,
  * compressed at creation (per-element neighbour table, deduplicated per-face
  * geometry); arrays the GPU path cannot represent (non-affine geometry, or
  * VMapM not addressing the element's own Fmask nodes) give
- * DGK_ERR_UNSUPPORTED.  Lift_mat is used as stored by "HIP"; "HIP_MP" uses
- * its separable 1D form and rejects a Lift_mat that is not separable to
- * 1e-10 relative. */
+ * DGK_ERR_UNSUPPORTED.  "HIP" applies Lift_mat in its separable 1D form
+ * (l-/l+ along each face normal) when every stored entry lies within 1e-12
+ * of max|Lift_mat| of that form, and reads Lift_mat as stored otherwise;
+ * "HIP_MP" always uses the 1D form and rejects a Lift_mat that deviates from
+ * it by more than 1e-10 relative. */
 int dgk_create(int evaltype, int Nq, int64_t Ne, int64_t NeA,
                int64_t NhaloNode, const double *D1D, const double *Lift_mat,
                const int *VMapM, const int *VMapP, const int *halo_src_map,
@@ -204,6 +206,83 @@ int dgk_batch_step(dgk_batch *b, int nstep);
 int dgk_batch_synchronize(dgk_batch *b);
 /* device seconds and number of dgk_batch_step calls since the last read */
 int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
+
+/* ---- serving: continuous batching of advection solves -------------------
+ * No counterpart in the reference, whose driver runs one mesh to completion
+ * (main.f90:51-79); these entry points are for callers that serve many
+ * solves at once.  A server owns one device arena of hbm_budget bytes, a
+ * worker thread and nstreams HIP streams (default priority).  A request is a
+ * periodic box with a Gaussian-blob initial condition and constant velocity
+ * (the inputs of dgk_init_gaussian).  Requests wait in arrival order and are
+ * admitted when their device footprint fits in the arena and one of
+ * max_active slots is free; a request may be passed by later ones that fit,
+ * at most 8 times, after which admission waits for it.  Every tick advances
+ * each admitted request by one SSP-RK3 step on its stream, so a request that
+ * arrives while others run joins at the next tick and its kernels overlap
+ * theirs.  A request runs the same stage kernels on the same inputs as a
+ * dgk_solver of its box, so its result does not depend on what else is
+ * admitted.  All dgk_serve_* calls may come from any thread. */
+enum { DGK_TOL_FP64 = 0, /* plain FP64 path (HIP)                         */
+       DGK_TOL_1E5 = 1   /* mixed precision (HIP_MP), 1e-5 relative L2     */ };
+
+typedef struct dgk_request {
+  int32_t ne[3];     /* elements per axis (>= 1)                           */
+  int32_t p;         /* polynomial order 1..15                             */
+  int32_t nstep;     /* SSP-RK3 steps (>= 0)                               */
+  int32_t tol_class; /* DGK_TOL_FP64 or DGK_TOL_1E5                        */
+  double dom[3];     /* box size per axis (> 0)                            */
+  double dt;         /* time step; <= 0: CFL 0.1 on the smallest node gap,
+                        dt = 0.1 * min_d (x1(1)-x1(0)) L_d/(2 Ne_d) / sum|vel| */
+  double vel[3];     /* constant velocity                                  */
+  double centre[3];  /* blob centre                                        */
+  double sigma;      /* blob width (> 0)                                   */
+  double *q_out;     /* optional host buffer of Np*Ne doubles that receives
+                        the final q before the result is posted (NULL: the
+                        summary only)                                      */
+  int64_t tag;       /* caller's label, returned in the result             */
+} dgk_request;
+
+typedef struct dgk_result {
+  int64_t id;        /* from dgk_serve_submit                              */
+  int64_t tag;
+  int32_t status;    /* DGK_OK or the failing call's code                  */
+  int32_t evaltype;  /* DGK_EVAL_HIP or DGK_EVAL_HIP_MP                    */
+  int32_t p, nstep;
+  int64_t dof;       /* Np*Ne                                              */
+  int64_t bytes;     /* device footprint                                   */
+  double dt;
+  double t_submit, t_admit, t_done; /* seconds since dgk_serve_create    */
+  double qmin, qmax; /* of the final q                                     */
+  double mass;       /* quadrature integral of the final q over the box    */
+  double l2sq;       /* quadrature integral of q^2                         */
+  char message[160]; /* error text when status != DGK_OK                   */
+} dgk_result;
+
+typedef struct dgk_serve_stats {
+  int64_t submitted, admitted, completed, failed;
+  int64_t queued, active, active_peak, ticks;
+  int64_t bytes_in_use, bytes_peak, budget;
+  int64_t bypasses;      /* admissions that passed a waiting request       */
+  double dof_updates;    /* sum of dof * nstep of completed requests       */
+} dgk_serve_stats;
+
+typedef struct dgk_server dgk_server;
+/* hbm_budget: arena bytes (allocated once); max_active >= 1; nstreams 1..8 */
+int dgk_serve_create(const char *operator_dir, int64_t hbm_budget, int max_active,
+                     int nstreams, dgk_server **out);
+/* Device bytes a request of this shape occupies while admitted. */
+int dgk_serve_footprint(const int32_t ne[3], int p, int tol_class, int64_t *bytes);
+/* Queue a request (copied).  DGK_ERR_NOMEM when its footprint exceeds the
+ * whole arena, DGK_ERR_ARG / DGK_ERR_IO for a malformed request or a missing
+ * operator file, DGK_ERR_STATE after a device failure of the worker. */
+int dgk_serve_submit(dgk_server *sv, const dgk_request *r, int64_t *id);
+/* Move up to max finished results into out (completion order); waits up to
+ * timeout_s seconds for the first one (0: do not wait). */
+int dgk_serve_poll(dgk_server *sv, dgk_result *out, int max, int *n, double timeout_s);
+int dgk_serve_get_stats(dgk_server *sv, dgk_serve_stats *st);
+/* Finish every submitted request, stop the worker, free the arena.
+ * Results not yet polled are discarded. */
+int dgk_serve_destroy(dgk_server *sv);
 
 #ifdef __cplusplus
 }

@@ -148,22 +148,22 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
 
 }  // namespace dgk
 
-namespace {
+namespace dgk {
 
 // Separable 1D form of the stored Lift: Lift(:,:,:,f) = I (x) l-/+ (x) I
 // along the face normal (faces -y,+x,+y,-x,-z,+z use l-(j), l+(i), l+(j),
 // l-(i), l-(k), l+(k)).  l-/+ are read from the -x and +x blocks at
 // tangential node (0,0); returns the largest deviation of any stored entry
 // from that form relative to max|Lift|.
-double separable_lift(const MeshTables &t, std::vector<double> &ell) {
-  const int Nq = t.Nq, Np = t.Np;
+double separable_lift(int Nq, const std::vector<double> &Lift, std::vector<double> &ell) {
+  const int Np = Nq * Nq * Nq;
   ell.assign(2 * Nq, 0.0);
   for (int a = 0; a < Nq; ++a) {
-    ell[a] = t.Lift[a + 3 * Np];       // face -x: Lift(a,0,0,4) = l-(a)
-    ell[Nq + a] = t.Lift[a + 1 * Np];  // face +x: Lift(a,0,0,2) = l+(a)
+    ell[a] = Lift[a + 3 * Np];       // face -x: Lift(a,0,0,4) = l-(a)
+    ell[Nq + a] = Lift[a + 1 * Np];  // face +x: Lift(a,0,0,2) = l+(a)
   }
   double scale = 0.0;
-  for (double x : t.Lift) scale = std::fmax(scale, std::fabs(x));
+  for (double x : Lift) scale = std::fmax(scale, std::fabs(x));
   double dev = 0.0;
   for (int f = 0; f < 6; ++f)
     for (int k = 0; k < Nq; ++k)
@@ -172,18 +172,12 @@ double separable_lift(const MeshTables &t, std::vector<double> &ell) {
           const int axis_idx = (f == 1 || f == 3) ? i : (f == 0 || f == 2) ? j : k;
           const bool plus = (f == 1 || f == 2 || f == 5);
           const double ref = ell[(plus ? Nq : 0) + axis_idx];
-          dev = std::fmax(dev, std::fabs(t.Lift[i + j * Nq + k * Nq * Nq + f * Np] - ref));
+          dev = std::fmax(dev, std::fabs(Lift[i + j * Nq + k * Nq * Nq + f * Np] - ref));
         }
   return scale > 0.0 ? dev / scale : dev;
 }
 
-// Lift_mat deviations from the tensor-product form below this (relative)
-// are rounding in the stored data (2.8e-14 for p = 1..15); the FP64 path then
-// applies the 1D form, otherwise it reads Lift_mat as stored.
-constexpr double kSeparableF64 = 1e-12;
-constexpr double kSeparableMP = 1e-10;
-
-}  // namespace
+}  // namespace dgk
 
 namespace dgk {
 
@@ -201,7 +195,7 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
     return set_error(DGK_ERR_DEVICE, "no HIP device available (the path has no CPU fallback)");
   }
   std::vector<double> ell;
-  const double sep_dev = separable_lift(t, ell);
+  const double sep_dev = separable_lift(t.Nq, t.Lift, ell);
   if (evaltype == DGK_EVAL_HIP_MP && !(sep_dev <= kSeparableMP))
     return set_error(DGK_ERR_UNSUPPORTED,
                      "Lift_mat is not of tensor-product form (relative deviation " +

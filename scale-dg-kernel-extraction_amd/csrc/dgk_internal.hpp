@@ -92,6 +92,8 @@ constexpr int kFaceAxis[6] = {1, 0, 1, 0, 2, 2};
 // True when every geometry record's face normals have zero components off
 // the face's axis.
 bool geo_axis_aligned(const std::vector<double> &geo);
+// Geometry record of an axis-aligned hexahedron of size dx x dy x dz.
+void fill_box_geo(double dx, double dy, double dz, double *rec);
 
 // Processing order of a NeX x NeY x NeZ box in bricks of bx x by x bz
 // elements (bricks x-fastest, elements x-fastest inside a brick).  Only the
@@ -201,5 +203,15 @@ int launch_init_gaussian(double *q, double *u, double *v, double *w, const doubl
                          const double c[3], double sigma, const double vel[3],
                          hipStream_t s);
 constexpr int kMinmaxBlocks = 1024;
+
+// Periodic neighbour table of a NeX x NeY x NeZ box built on the device, with
+// element indices offset by elem_off.
+int launch_box_nbr(int32_t *nbr, int NeX, int NeY, int NeZ, int64_t elem_off,
+                   hipStream_t s);
+// result[0..3] = min q, max q, jac * sum w3[n % Np] q[n], jac * sum w3[n % Np] q[n]^2
+// (partial: 4 * kSummaryBlocks doubles).
+constexpr int kSummaryBlocks = 256;
+int launch_field_summary(const double *q, const double *w3, int Np, int64_t n, double jac,
+                         double *partial, double *result, hipStream_t s);
 
 }  // namespace dgk

@@ -64,16 +64,6 @@ int64_t neighbour(int i, int j, int k, int f, int NeX, int NeY, int NeZ,
   return i + (int64_t)j * NeX + (int64_t)k * NeX * NeY;
 }
 
-void fill_box_geo(double dx, double dy, double dz, double *rec) {
-  const double h[3] = {dx, dy, dz};
-  std::memset(rec, 0, sizeof(double) * kGeoStride);
-  for (int d = 0; d < 3; ++d) rec[kGeoEscale + d] = 2.0 / h[d];
-  for (int f = 0; f < 6; ++f) {
-    rec[kGeoFscale + f] = 2.0 / h[kFaceAxis[f]];
-    rec[kGeoNormal + 3 * f + kFaceAxis[f]] = kFaceSign[f];
-  }
-}
-
 struct RecHash {
   size_t operator()(const std::array<uint64_t, 27> &a) const {
     uint64_t h = 1469598103934665603ULL;
@@ -83,6 +73,16 @@ struct RecHash {
 };
 
 }  // namespace
+
+void fill_box_geo(double dx, double dy, double dz, double *rec) {
+  const double h[3] = {dx, dy, dz};
+  std::memset(rec, 0, sizeof(double) * kGeoStride);
+  for (int d = 0; d < 3; ++d) rec[kGeoEscale + d] = 2.0 / h[d];
+  for (int f = 0; f < 6; ++f) {
+    rec[kGeoFscale + f] = 2.0 / h[kFaceAxis[f]];
+    rec[kGeoNormal + 3 * f + kFaceAxis[f]] = kFaceSign[f];
+  }
+}
 
 bool geo_axis_aligned(const std::vector<double> &geo) {
   for (size_t r = 0; r + kGeoStride <= geo.size(); r += kGeoStride)

@@ -91,4 +91,14 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
                  int dt_geo = 0);
 int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out);
 
+// Separable 1D form of a stored Lift (Nq^3*6): ell = l-(0..Nq-1), l+(0..Nq-1);
+// returns the largest deviation of a stored entry from that form relative to
+// max|Lift|.
+double separable_lift(int Nq, const std::vector<double> &Lift, std::vector<double> &ell);
+// Lift deviations from the tensor-product form below this (relative) are
+// rounding in the stored data (2.8e-14 for p = 1..15); the FP64 path then
+// applies the 1D form, otherwise it reads Lift as stored.
+constexpr double kSeparableF64 = 1e-12;
+constexpr double kSeparableMP = 1e-10;
+
 }  // namespace dgk

@@ -73,6 +73,12 @@ _SIGNATURES = {
     "dgk_batch_step": (_I, [_P, _I]),
     "dgk_batch_synchronize": (_I, [_P]),
     "dgk_batch_timer": (_I, [_P, _P, _P]),
+    "dgk_serve_create": (_I, [_S, _L, _I, _I, _P]),
+    "dgk_serve_footprint": (_I, [_P, _I, _I, _P]),
+    "dgk_serve_submit": (_I, [_P, _P, _P]),
+    "dgk_serve_poll": (_I, [_P, _P, _I, _P, _D]),
+    "dgk_serve_get_stats": (_I, [_P, _P]),
+    "dgk_serve_destroy": (_I, [_P]),
 }
 
 _lib = None

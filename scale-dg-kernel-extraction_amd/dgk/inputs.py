@@ -184,3 +184,34 @@ def lgl_nodes(p):
     if p not in _LGL:
         _LGL[p] = load_operators(p)[0]
     return _LGL[p]
+
+
+def serve_requests(n=10000, seed=5, rate=4.0, ne_range=(4, 32), p_range=(2, 15),
+                   nstep_range=(10, 500), sigma_range=(0.05, 0.2), mp_fraction=0.5):
+    """Serving stream: request k draws, from counters of seed `seed` starting at
+    PARAM_BASE + 16k, Ne per axis independently and uniformly from ne_range
+    (inclusive), p from p_range, a velocity with components U[-1,1], a
+    Gaussian-blob centre U[0,1)^3, sigma U[sigma_range], nstep from
+    nstep_range and the tolerance class (1e-5 with probability mp_fraction,
+    else FP64); its arrival follows request k-1 after an exponential gap of
+    mean 1/rate seconds (a Poisson stream of rate `rate`).  dt is left to the
+    server (CFL 0.1)."""
+    out = []
+    t = 0.0
+
+    def pick(u, lo, hi):
+        return int(lo + min(int(u * (hi - lo + 1)), hi - lo))
+
+    for k in range(n):
+        r = u01(seed, PARAM_BASE + 16 * k + np.arange(14, dtype=np.uint64))
+        t += -np.log1p(-float(r[13])) / rate
+        out.append(dict(
+            ne=tuple(pick(float(r[c]), *ne_range) for c in range(3)),
+            p=pick(float(r[3]), *p_range),
+            vel=tuple(float(2.0 * r[4 + c] - 1.0) for c in range(3)),
+            centre=tuple(float(r[7 + c]) for c in range(3)),
+            sigma=float(sigma_range[0] + (sigma_range[1] - sigma_range[0]) * r[10]),
+            nstep=pick(float(r[11]), *nstep_range),
+            tol_class=1 if float(r[12]) < mp_fraction else 0,
+            arrival=float(t)))
+    return out

@@ -102,3 +102,40 @@ def test_input_generator_matches_oracle_stream():
         ours = inputs.u01(seed, c)
         ref = [oracle.u01(seed, int(x)) for x in c]
         assert np.array_equal(ours, np.array(ref))
+
+
+def test_serve_footprint_and_no_device():
+    from dgk import serving
+
+    # q (3 buffers) + u, v, w in FP64, plus FP32 velocity copies for the 1e-5
+    # class, neighbour table, geometry record and summary partials, rounded
+    # to 64 KiB blocks
+    for ne, p, cls in (((4, 4, 4), 2, 0), ((32, 32, 32), 15, 1), ((5, 7, 3), 7, 0)):
+        ne_tot = int(np.prod(ne))
+        dof = ne_tot * (p + 1) ** 3
+        fields = 6 * 8 * dof + (3 * 4 * dof if cls else 0)
+        got = serving.footprint(ne, p, cls)
+        assert got % 65536 == 0
+        assert fields + 24 * ne_tot <= got <= fields + 24 * ne_tot + 8 * 256 + 8224 + 65536 + 256 * 10
+    with pytest.raises(dgk.DgkError) as ei:
+        serving.footprint((4, 4, 4), 16, 0)
+    assert ei.value.code == dgk.ERR_UNSUPPORTED
+    if dgk.device_count() == 0:
+        with pytest.raises(dgk.DgkError) as ei:
+            serving.Server(hbm_budget=1 << 20)
+        assert ei.value.code == dgk.ERR_DEVICE
+
+
+def test_serve_request_stream_ranges():
+    reqs = inputs.serve_requests(n=3000, seed=5, rate=4.0)
+    assert reqs == inputs.serve_requests(n=3000, seed=5, rate=4.0)
+    ne = np.array([r["ne"] for r in reqs])
+    assert ne.min() == 4 and ne.max() == 32
+    p = np.array([r["p"] for r in reqs])
+    assert p.min() == 2 and p.max() == 15 and len(set(p)) == 14
+    ns = np.array([r["nstep"] for r in reqs])
+    assert ns.min() >= 10 and ns.max() <= 500
+    gaps = np.diff([0.0] + [r["arrival"] for r in reqs])
+    assert np.all(gaps > 0) and abs(gaps.mean() - 0.25) < 0.02  # Poisson, rate 4/s
+    frac = np.mean([r["tol_class"] for r in reqs])
+    assert 0.45 < frac < 0.55
