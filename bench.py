@@ -18,7 +18,12 @@ The JSON line also carries
                 ensemble (512 seeded instances, Ne/axis in {4,6,8,12,16},
                 p in {3,5,7,11}, 50 steps, graph-replayed) in FP64 and mixed
                 precision against the plain FP64 path run instance by
-                instance; serving latency when --serve is given.
+                instance; serving: the first --serve requests of the seeded
+                Poisson stream (Ne/axis 4..32, p 2..15, nstep 10..500, half
+                of them in the 1e-5 class) at --serve-rate requests/s, with
+                p50/p99 latency, DOF-updates/s and a sample of results checked
+                against the same solves alone, plus a burst of --serve-burst
+                requests for the serving capacity.
 """
 from __future__ import annotations
 
@@ -49,8 +54,12 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=120,
                     help="time steps of the bounded CPU baseline sample")
     ap.add_argument("--no-extra", action="store_true", help="headline only")
-    ap.add_argument("--serve", type=int, default=0,
-                    help="serve this many requests of the seeded Poisson stream")
+    ap.add_argument("--serve", type=int, default=150,
+                    help="requests of the seeded Poisson stream to serve (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=6.0,
+                    help="arrival rate of the serving stream, requests/s")
+    ap.add_argument("--serve-burst", type=int, default=60,
+                    help="requests submitted at once to measure serving capacity (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (the reference "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -239,7 +248,15 @@ def main():
     if args.serve:
         from dgk import serving
 
-        line["extra"]["serving"] = serving.run_stream(args.serve)
+        sv = serving.run_stream(n=args.serve, rate=args.serve_rate)
+        if args.serve_burst:
+            cap = serving.run_stream(n=args.serve_burst, rate=1e6, sample=0)
+            sv["capacity_burst"] = {k: cap[k] for k in ("requests", "DOF-updates/s",
+                                                        "makespan_s", "p50_latency_s",
+                                                        "p99_latency_s")}
+        line["extra"]["serving"] = sv
+        line["extra"]["p50_latency_s"] = sv["p50_latency_s"]
+        line["extra"]["p99_latency_s"] = sv["p99_latency_s"]
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)

@@ -217,9 +217,9 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  * admitted when their device footprint fits in the arena and one of
  * max_active slots is free; a request may be passed by later ones that fit,
  * at most 8 times, after which admission waits for it.  Every tick advances
- * each admitted request by one SSP-RK3 step on its stream, so a request that
- * arrives while others run joins at the next tick and its kernels overlap
- * theirs.  A request runs the same stage kernels on the same inputs as a
+ * each admitted request by SSP-RK3 steps on its stream (see
+ * dgk_serve_set_policy), so a request that arrives while others run joins at
+ * the next tick and its kernels overlap theirs.  A request runs the same stage kernels on the same inputs as a
  * dgk_solver of its box, so its result does not depend on what else is
  * admitted.  All dgk_serve_* calls may come from any thread. */
 enum { DGK_TOL_FP64 = 0, /* plain FP64 path (HIP)                         */
@@ -279,6 +279,13 @@ int dgk_serve_submit(dgk_server *sv, const dgk_request *r, int64_t *id);
 /* Move up to max finished results into out (completion order); waits up to
  * timeout_s seconds for the first one (0: do not wait). */
 int dgk_serve_poll(dgk_server *sv, dgk_result *out, int max, int *n, double timeout_s);
+/* Tick policy.  STEP_FAIR: one step per admitted request per tick.
+ * TIME_FAIR (the default, max_steps_per_tick 64): about the same device time
+ * per request per tick; a request of D DOF gets Q / max(D, 2^18) steps,
+ * 1..max_steps_per_tick, with Q the largest max(D, 2^18) among the admitted
+ * requests and at least 2^22.  Results do not depend on the policy. */
+enum { DGK_SERVE_STEP_FAIR = 0, DGK_SERVE_TIME_FAIR = 1 };
+int dgk_serve_set_policy(dgk_server *sv, int policy, int max_steps_per_tick);
 int dgk_serve_get_stats(dgk_server *sv, dgk_serve_stats *st);
 /* Finish every submitted request, stop the worker, free the arena.
  * Results not yet polled are discarded. */

@@ -11,16 +11,21 @@
 //  * admission arrival order, a request enters when its block fits and one of
 //             max_active slots is free; later requests that fit may pass a
 //             waiting one at most kMaxBypass times.
-//  * ticks    every tick enqueues one SSP-RK3 step (three fused stage
-//             launches, the kernels of dgk_step) for every admitted request
-//             on that request's stream, streams chosen by least resident DOF.
-//             At most two ticks are in flight; the worker retires a tick's
-//             finished requests when its events complete and admits the
-//             arrivals that came meanwhile, so new work joins a running mix.
+//  * ticks    every tick enqueues SSP-RK3 steps (three fused stage launches
+//             each, the kernels of dgk_step) for every admitted request on
+//             that request's stream, streams chosen by least resident DOF:
+//             one step each (step-fair), or, by default, about the same
+//             device time each (time-fair: a request gets the largest step
+//             cost of the tick over its own step cost, in steps, so small
+//             solves are not paced by large ones).  At most two ticks are in
+//             flight; the worker retires a tick's finished requests when its
+//             events complete and admits the arrivals that came meanwhile, so
+//             new work joins a running mix.
 //  * results  a device summary (min, max, quadrature mass and norm) per
 //             request, optionally the whole final field, posted with submit,
 //             admission and completion times.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -46,6 +51,12 @@ constexpr int kMaxStreams = 8;
 constexpr int64_t kGranule = 1 << 16;  // arena block granularity
 constexpr int64_t kAlign = 256;        // sub-array alignment inside a block
 constexpr int kSlotDoubles = 4 + kGeoStride;  // pinned: summary + geometry record
+// Time-fair ticks: a step's cost is its DOF, at least kCostFloor (the launch
+// latency of three stage kernels is worth about that many DOF); the tick's
+// quantum is the largest cost among the admitted requests, at least
+// kMinQuantum.
+constexpr int64_t kCostFloor = 1 << 18;
+constexpr int64_t kMinQuantum = 1 << 22;
 
 int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
@@ -183,6 +194,8 @@ struct dgk_server {
   std::chrono::steady_clock::time_point t0;
   Arena arena;
   int max_active = 0;
+  std::atomic<int> policy{DGK_SERVE_TIME_FAIR};
+  std::atomic<int> max_steps{64};
   std::vector<hipStream_t> streams;
   hipStream_t copy_stream = nullptr;
   double *pinned = nullptr;  // max_active slots of kSlotDoubles
@@ -371,10 +384,15 @@ int finish(dgk_server *sv, Req *r, hipStream_t stream) {
   return DGK_OK;
 }
 
-// One SSP-RK3 step of every admitted request (buffer rotation of dgk_step:
-// stages read A, B, C and write B, C, A, with A the RK base).
+// SSP-RK3 steps of every admitted request (buffer rotation of dgk_step:
+// stages read A, B, C and write B, C, A, with A the RK base), as many per
+// request as the tick policy gives it.
 int enqueue_tick(dgk_server *sv) {
   if (sv->active.empty()) return DGK_OK;
+  const bool time_fair = sv->policy.load() == DGK_SERVE_TIME_FAIR;
+  const int max_steps = std::max(1, sv->max_steps.load());
+  int64_t quantum = kMinQuantum;
+  for (const Req *r : sv->active) quantum = std::max(quantum, std::max(r->dof, kCostFloor));
   dgk_server::Tick t;
   t.ev.assign(sv->streams.size(), nullptr);
   std::vector<char> used(sv->streams.size(), 0);
@@ -383,11 +401,15 @@ int enqueue_tick(dgk_server *sv) {
   for (Req *r : sv->active) {
     hipStream_t stream = sv->streams[r->stream];
     used[r->stream] = 1;
-    if (r->steps_left > 0) {
-      dgk_solver &s = r->s;
-      double *A = s.q[0], *B = s.q[1], *C = s.q[2];
-      const double *in[3] = {A, B, C};
-      double *out[3] = {B, C, A};
+    int n = 1;
+    if (time_fair)
+      n = (int)std::min<int64_t>(max_steps, quantum / std::max(r->dof, kCostFloor));
+    n = std::max(1, std::min(n, r->steps_left));
+    dgk_solver &s = r->s;
+    double *A = s.q[0], *B = s.q[1], *C = s.q[2];
+    const double *in[3] = {A, B, C};
+    double *out[3] = {B, C, A};
+    for (int k = 0; k < n && r->steps_left > 0; ++k) {
       for (int st = 0; st < 3; ++st)
         DGK_TRY(launch_stage(&s, in[st], A, out[st], 1, kRkA[st], kRkB[st], r->dt, stream, 0));
       r->steps_left--;
@@ -703,6 +725,18 @@ int dgk_serve_poll(dgk_server *sv, dgk_result *out, int max, int *n, double time
     out[(*n)++] = sv->results.front();
     sv->results.pop_front();
   }
+  return DGK_OK;
+}
+
+int dgk_serve_set_policy(dgk_server *sv, int policy, int max_steps_per_tick) {
+  clear_error();
+  if (!sv) return set_error(DGK_ERR_ARG, "server handle is NULL");
+  if (policy != DGK_SERVE_STEP_FAIR && policy != DGK_SERVE_TIME_FAIR)
+    return set_error(DGK_ERR_ARG, "unknown tick policy " + std::to_string(policy));
+  if (max_steps_per_tick < 1 || max_steps_per_tick > 1024)
+    return set_error(DGK_ERR_ARG, "max_steps_per_tick must be 1..1024");
+  sv->max_steps.store(max_steps_per_tick);
+  sv->policy.store(policy);
   return DGK_OK;
 }
 

@@ -77,6 +77,7 @@ _SIGNATURES = {
     "dgk_serve_footprint": (_I, [_P, _I, _I, _P]),
     "dgk_serve_submit": (_I, [_P, _P, _P]),
     "dgk_serve_poll": (_I, [_P, _P, _I, _P, _D]),
+    "dgk_serve_set_policy": (_I, [_P, _I, _I]),
     "dgk_serve_get_stats": (_I, [_P, _P]),
     "dgk_serve_destroy": (_I, [_P]),
 }

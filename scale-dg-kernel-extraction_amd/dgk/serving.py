@@ -19,6 +19,7 @@ from . import (EVAL_HIP, EVAL_HIP_MP, FIELD_Q, OPERATOR_DIR, Solver, _check, _pt
 from . import inputs
 
 TOL_FP64, TOL_1E5 = 0, 1
+STEP_FAIR, TIME_FAIR = 0, 1
 
 
 class Request(C.Structure):
@@ -66,12 +67,17 @@ class Server:
     max_active: requests resident at once; nstreams: HIP streams."""
 
     def __init__(self, hbm_budget: int = 16 << 30, max_active: int = 64, nstreams: int = 4,
+                 policy: int = TIME_FAIR, max_steps_per_tick: int = 64,
                  operator_dir=OPERATOR_DIR):
         h = C.c_void_p()
         _check(lib().dgk_serve_create(str(operator_dir).encode(), int(hbm_budget),
                                       int(max_active), int(nstreams), C.byref(h)))
         self._h = h
         self._outputs = {}  # id -> host array receiving the final q
+        self.set_policy(policy, max_steps_per_tick)
+
+    def set_policy(self, policy: int, max_steps_per_tick: int = 64) -> None:
+        _check(lib().dgk_serve_set_policy(self._h, int(policy), int(max_steps_per_tick)))
 
     def submit(self, req: dict, q_out: np.ndarray | None = None, tag: int = 0) -> int:
         """Queue one request (keys ne, p, nstep, tol_class, vel, centre, sigma,
@@ -156,7 +162,8 @@ def _pct(x, q):
 
 def run_stream(n: int = 200, rate: float = 4.0, seed: int = 5, hbm_budget: int = 64 << 30,
                max_active: int = 64, nstreams: int = 4, sample: int = 8,
-               sample_max_dof: int = 4_000_000, progress: bool = True) -> dict:
+               sample_max_dof: int = 4_000_000, progress: bool = True,
+               policy: int = TIME_FAIR, max_steps_per_tick: int = 64) -> dict:
     """Replay the first n requests of the seeded Poisson stream in real time.
 
     Up to `sample` requests of each tolerance class with at most
@@ -172,7 +179,8 @@ def run_stream(n: int = 200, rate: float = 4.0, seed: int = 5, hbm_budget: int =
             picked[r["tol_class"]].append(k)
     wanted = set(picked[TOL_FP64] + picked[TOL_1E5])
     results = {}
-    with Server(hbm_budget=hbm_budget, max_active=max_active, nstreams=nstreams) as sv:
+    with Server(hbm_budget=hbm_budget, max_active=max_active, nstreams=nstreams,
+                policy=policy, max_steps_per_tick=max_steps_per_tick) as sv:
         t0 = time.perf_counter()
         nxt, last_note = 0, t0
         while len(results) < n:
@@ -214,6 +222,8 @@ def run_stream(n: int = 200, rate: float = 4.0, seed: int = 5, hbm_budget: int =
         "server": {k: stats[k] for k in ("active_peak", "bytes_peak", "budget", "ticks",
                                           "bypasses")},
         "hbm_budget": hbm_budget, "max_active": max_active, "nstreams": nstreams,
+        "policy": "time-fair" if policy == TIME_FAIR else "step-fair",
+        "max_steps_per_tick": max_steps_per_tick,
     }
     for cls, name in ((TOL_FP64, "fp64"), (TOL_1E5, "1e-5")):
         sel = [lat[k] for k in range(n) if reqs[k]["tol_class"] == cls]

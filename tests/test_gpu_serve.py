@@ -54,9 +54,12 @@ def serve_all(reqs, **kw):
     return outs, [by_tag[k] for k in range(len(reqs))], st
 
 
-def test_served_results_equal_solves_alone():
+@pytest.mark.parametrize("policy,max_steps", [(serving.TIME_FAIR, 16), (serving.STEP_FAIR, 1),
+                                              (serving.TIME_FAIR, 3)])
+def test_served_results_equal_solves_alone(policy, max_steps):
     reqs = small_requests(12, seed=21)
-    outs, res, st = serve_all(reqs, hbm_budget=1 << 30, max_active=5, nstreams=3)
+    outs, res, st = serve_all(reqs, hbm_budget=1 << 30, max_active=5, nstreams=3,
+                              policy=policy, max_steps_per_tick=max_steps)
     assert st["completed"] == len(reqs) and st["active_peak"] <= 5
     for k, r in enumerate(reqs):
         alone = serving.solve_alone(r, res[k]["form"], res[k]["dt"])
