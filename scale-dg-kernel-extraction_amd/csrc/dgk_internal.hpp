@@ -186,6 +186,11 @@ struct StageArgsMP {
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
+// Tile shapes (elements per block) of the mixed-precision stage kernel of
+// order Nq: fills epb[0..2] and returns 3.  stage_mp_select picks the
+// candidate that later launches of that Nq use.
+int stage_mp_shapes(int Nq, int *epb, int n);
+int stage_mp_select(int Nq, int cand);
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,
                             const double *v, const double *w,
                             const int32_t *nbr, const int32_t *vmapP,
