@@ -207,6 +207,28 @@ int dgk_batch_synchronize(dgk_batch *b);
 /* device seconds and number of dgk_batch_step calls since the last read */
 int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
 
+/* ---- tile shapes and the autotuner --------------------------------------
+ * The stage kernels come in one or more tile shapes per order: the number of
+ * elements one workgroup advances (HIP_MP: one, two or four wavefronts of
+ * whole elements for p <= 7, one or two elements above; HIP: one shape).
+ * Candidate 0 is the default.  The selection holds for every later launch
+ * of that form and order in the process.  dgk_autotune times each candidate
+ * with HIP events over reps RK-stage launches on the solver's own mesh
+ * (writing only the solver's scratch rotation buffer) and selects the
+ * fastest; us_per_launch[c] receives candidate c's time.  dgk_batch_autotune
+ * does this for every order group of a batch and reports per group: the
+ * order, the chosen candidate, its time and the default's time (the batch's
+ * captured step is re-captured at the next dgk_batch_step).  No counterpart
+ * in the reference, whose OPT1 kernels are generated per order
+ * (mod_dg_optr_kernel_opt1.F90.erb); results do not depend on the shape. */
+int dgk_tile_shapes(int evaltype, int p, int *epb, int max, int *n);
+int dgk_tile_select(int evaltype, int p, int cand);
+int dgk_tile_selected(int evaltype, int p, int *cand);
+int dgk_autotune(dgk_solver *s, int reps, int *best, double *us_per_launch, int max,
+                 int *n);
+int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_best,
+                       double *us_default, int max, int *n);
+
 /* ---- serving: continuous batching of advection solves -------------------
  * No counterpart in the reference, whose driver runs one mesh to completion
  * (main.f90:51-79); these entry points are for callers that serve many

@@ -906,3 +906,146 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Tile shapes and the autotuner.  A stage kernel has one or more tile shapes
+// per order (elements per workgroup); the selection is per process, per form
+// and order.  The autotuner times every candidate on a solver's own mesh with
+// HIP events around `reps` RK-stage launches (stage-2 configuration: RK base
+// read, output to the solver's scratch rotation buffer) and keeps the
+// fastest.  Per-element arithmetic does not depend on the shape.
+// ---------------------------------------------------------------------------
+namespace {
+
+int tile_shapes(int evaltype, int Nq, int *epb) {
+  if (evaltype == DGK_EVAL_HIP_MP) return stage_mp_shapes(Nq, epb, 3);
+  if (evaltype == DGK_EVAL_HIP) return stage_f64_shapes(Nq, epb, 3);
+  return 0;
+}
+
+int check_form_order(int evaltype, int p) {
+  if (evaltype != DGK_EVAL_HIP && evaltype != DGK_EVAL_HIP_MP)
+    return set_error(DGK_ERR_UNSUPPORTED, "tile shapes exist for HIP and HIP_MP only");
+  if (p < kMinNq - 1 || p > kMaxNq - 1)
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+  return DGK_OK;
+}
+
+int tile_select(int evaltype, int Nq, int cand) {
+  if (evaltype == DGK_EVAL_HIP_MP) return stage_mp_select(Nq, cand);
+  if (cand != 0) return set_error(DGK_ERR_ARG, "the FP64 stage kernel has one tile shape");
+  return DGK_OK;
+}
+
+int tile_selected(int evaltype, int Nq) {
+  return evaltype == DGK_EVAL_HIP_MP ? stage_mp_selected(Nq) : 0;
+}
+
+// Average device time of one stage launch of s, in microseconds.
+int time_stage(dgk_solver *s, int reps, double *us) {
+  double *A = s->q[s->cur], *C = s->q[(s->cur + 2) % 3];
+  hipEvent_t a = nullptr, b = nullptr;
+  DGK_HIP(hipEventCreate(&a));
+  DGK_HIP(hipEventCreate(&b));
+  int rc = launch_stage(s, A, A, C, 1, kRkA[1], kRkB[1], 0.0);
+  if (rc == DGK_OK && hipEventRecord(a, s->stream) != hipSuccess) rc = set_error(DGK_ERR_DEVICE, "event");
+  for (int r = 0; r < reps && rc == DGK_OK; ++r) rc = launch_stage(s, A, A, C, 1, kRkA[1], kRkB[1], 0.0);
+  if (rc == DGK_OK && hipEventRecord(b, s->stream) != hipSuccess) rc = set_error(DGK_ERR_DEVICE, "event");
+  if (rc == DGK_OK && hipEventSynchronize(b) != hipSuccess) rc = set_error(DGK_ERR_DEVICE, "event sync");
+  float ms = 0.f;
+  if (rc == DGK_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = set_error(DGK_ERR_DEVICE, "elapsed");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc == DGK_OK) *us = 1e3 * ms / reps;
+  return rc;
+}
+
+int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
+  int epb[3];
+  const int nc = tile_shapes(s->evaltype, s->Nq, epb);
+  if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "no tile shapes for this form and order");
+  int arg = 0;
+  double t_best = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    DGK_TRY(tile_select(s->evaltype, s->Nq, c));
+    double t = 0.0;
+    DGK_TRY(time_stage(s, reps, &t));
+    if (us && c < max) us[c] = t;
+    if (c == 0 || t < t_best) {
+      t_best = t;
+      arg = c;
+    }
+  }
+  DGK_TRY(tile_select(s->evaltype, s->Nq, arg));
+  if (best) *best = arg;
+  if (n) *n = nc;
+  return DGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgk_tile_shapes(int evaltype, int p, int *epb, int max, int *n) {
+  clear_error();
+  DGK_TRY(check_form_order(evaltype, p));
+  if (!n || (max > 0 && !epb)) return set_error(DGK_ERR_ARG, "NULL argument");
+  int e[3];
+  const int nc = tile_shapes(evaltype, p + 1, e);
+  for (int c = 0; c < nc && c < max; ++c) epb[c] = e[c];
+  *n = nc;
+  return DGK_OK;
+}
+
+int dgk_tile_select(int evaltype, int p, int cand) {
+  clear_error();
+  DGK_TRY(check_form_order(evaltype, p));
+  return tile_select(evaltype, p + 1, cand);
+}
+
+int dgk_tile_selected(int evaltype, int p, int *cand) {
+  clear_error();
+  DGK_TRY(check_form_order(evaltype, p));
+  if (!cand) return set_error(DGK_ERR_ARG, "NULL argument");
+  *cand = tile_selected(evaltype, p + 1);
+  return DGK_OK;
+}
+
+int dgk_autotune(dgk_solver *s, int reps, int *best, double *us_per_launch, int max, int *n) {
+  DGK_TRY(enter(s));
+  DGK_TRY(require_fields(s));
+  if (reps < 1) return set_error(DGK_ERR_ARG, "reps must be >= 1");
+  return autotune(s, reps, best, us_per_launch, max, n);
+}
+
+int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_best,
+                       double *us_default, int max, int *n) {
+  DGK_TRY(batch_enter(b));
+  if (reps < 1) return set_error(DGK_ERR_ARG, "reps must be >= 1");
+  for (dgk_solver *s : b->groups) DGK_HIP(hipStreamSynchronize(s->stream));
+  for (size_t g = 0; g < b->groups.size(); ++g) {
+    dgk_solver *s = b->groups[g];
+    double t[3] = {0, 0, 0};
+    int arg = 0, nc = 0;
+    DGK_TRY(autotune(s, reps, &arg, t, 3, &nc));
+    if ((int)g < max) {
+      if (p) p[g] = b->group_p[g];
+      if (best) best[g] = arg;
+      if (us_best) us_best[g] = t[arg];
+      if (us_default) us_default[g] = t[0];
+    }
+  }
+  if (n) *n = (int)b->groups.size();
+  // the captured step used the previous shapes
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return DGK_OK;
+}
+
+}  // extern "C"

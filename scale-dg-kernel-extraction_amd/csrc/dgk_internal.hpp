@@ -191,6 +191,9 @@ int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
 // candidate that later launches of that Nq use.
 int stage_mp_shapes(int Nq, int *epb, int n);
 int stage_mp_select(int Nq, int cand);
+int stage_mp_selected(int Nq);
+// The FP64 stage kernel has one tile shape per order (fills epb[0], returns 1).
+int stage_f64_shapes(int Nq, int *epb, int n);
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,
                             const double *v, const double *w,
                             const int32_t *nbr, const int32_t *vmapP,

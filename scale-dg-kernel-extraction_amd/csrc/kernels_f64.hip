@@ -613,6 +613,19 @@ int prepare_stage_f64(int Nq) {
   }
 }
 
+int stage_f64_shapes(int Nq, int *epb, int n) {
+  if (!epb || n < 1) return 0;
+  switch (Nq) {
+#define DGK_F64_SHAPE(N) \
+  case N: epb[0] = Staged<N>::EPB; return 1;
+  DGK_F64_SHAPE(2) DGK_F64_SHAPE(3) DGK_F64_SHAPE(4) DGK_F64_SHAPE(5) DGK_F64_SHAPE(6)
+  DGK_F64_SHAPE(7) DGK_F64_SHAPE(8) DGK_F64_SHAPE(9) DGK_F64_SHAPE(10) DGK_F64_SHAPE(11)
+  DGK_F64_SHAPE(12) DGK_F64_SHAPE(13) DGK_F64_SHAPE(14) DGK_F64_SHAPE(15) DGK_F64_SHAPE(16)
+#undef DGK_F64_SHAPE
+  default: return 0;
+  }
+}
+
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,
                             const double *v, const double *w,
                             const int32_t *nbr, const int32_t *vmapP,

@@ -369,14 +369,20 @@ int stage_mp_shapes(int Nq, int *epb, int n) {
   DGK_MP_SHAPES(12) DGK_MP_SHAPES(13) DGK_MP_SHAPES(14) DGK_MP_SHAPES(15) DGK_MP_SHAPES(16)
 #undef DGK_MP_SHAPES
   }
-  return 3;
+  return Nq * Nq <= 64 ? 3 : 2;
 }
 
 int stage_mp_select(int Nq, int cand) {
   if (Nq < kMinNq || Nq > kMaxNq) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
-  if (cand < 0 || cand > 2) return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0..2");
+  const int nc = Nq * Nq <= 64 ? 3 : 2;
+  if (cand < 0 || cand >= nc)
+    return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
   g_mp_shape[Nq] = cand;
   return DGK_OK;
+}
+
+int stage_mp_selected(int Nq) {
+  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_mp_shape[Nq];
 }
 
 }  // namespace dgk

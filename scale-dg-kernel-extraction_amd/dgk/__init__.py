@@ -73,6 +73,11 @@ _SIGNATURES = {
     "dgk_batch_step": (_I, [_P, _I]),
     "dgk_batch_synchronize": (_I, [_P]),
     "dgk_batch_timer": (_I, [_P, _P, _P]),
+    "dgk_tile_shapes": (_I, [_I, _I, _P, _I, _P]),
+    "dgk_tile_select": (_I, [_I, _I, _I]),
+    "dgk_tile_selected": (_I, [_I, _I, _P]),
+    "dgk_autotune": (_I, [_P, _I, _P, _P, _I, _P]),
+    "dgk_batch_autotune": (_I, [_P, _I, _P, _P, _P, _P, _I, _P]),
     "dgk_serve_create": (_I, [_S, _L, _I, _I, _P]),
     "dgk_serve_footprint": (_I, [_P, _I, _I, _P]),
     "dgk_serve_submit": (_I, [_P, _P, _P]),
@@ -136,6 +141,29 @@ def evaltype_id(name: str) -> int:
     i = C.c_int(0)
     _check(lib().dgk_evaltype_id(name.encode(), C.byref(i)))
     return i.value
+
+
+def tile_shapes(evaltype, p: int) -> list:
+    """Elements per workgroup of each tile-shape candidate of a stage kernel."""
+    if isinstance(evaltype, str):
+        evaltype = evaltype_id(evaltype)
+    e, n = np.zeros(3, np.int32), C.c_int(0)
+    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 3, C.byref(n)))
+    return e[:n.value].tolist()
+
+
+def tile_select(evaltype, p: int, cand: int) -> None:
+    if isinstance(evaltype, str):
+        evaltype = evaltype_id(evaltype)
+    _check(lib().dgk_tile_select(int(evaltype), int(p), int(cand)))
+
+
+def tile_selected(evaltype, p: int) -> int:
+    if isinstance(evaltype, str):
+        evaltype = evaltype_id(evaltype)
+    c = C.c_int(0)
+    _check(lib().dgk_tile_selected(int(evaltype), int(p), C.byref(c)))
+    return c.value
 
 
 def element_sizes(p: int) -> dict:
@@ -314,6 +342,14 @@ class Solver:
         _check(lib().dgk_init_gaussian(self._h, *(float(c) for c in centre), float(sigma),
                                        *(float(c) for c in vel)))
 
+    def autotune(self, reps: int = 10) -> dict:
+        """Time every tile shape of this solver's form and order on its mesh and
+        select the fastest (process-wide for that form and order)."""
+        best, n = C.c_int(0), C.c_int(0)
+        us = np.zeros(3)
+        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 3, C.byref(n)))
+        return dict(best=best.value, us_per_launch=us[:n.value].tolist())
+
 
 class Batch:
     """Independent periodic boxes advanced together (dgk_batch_*).
@@ -334,6 +370,7 @@ class Batch:
                                       str(operator_dir).encode(), C.byref(h)))
         self._h = h
         self.n = n
+        self.evaltype = evaltype
 
     def close(self) -> None:
         if self._h:
@@ -390,3 +427,20 @@ class Batch:
         sec, calls = C.c_double(0), C.c_int64(0)
         _check(lib().dgk_batch_timer(self._h, C.byref(sec), C.byref(calls)))
         return sec.value, calls.value
+
+    def autotune(self, reps: int = 10) -> list:
+        """Autotune the tile shape of every order group; returns one record per
+        group: order, chosen elements per workgroup, its and the default's
+        microseconds per stage launch."""
+        m = 16
+        p, best, n = np.zeros(m, np.int32), np.zeros(m, np.int32), C.c_int(0)
+        ub, ud = np.zeros(m), np.zeros(m)
+        _check(lib().dgk_batch_autotune(self._h, int(reps), _ptr(p), _ptr(best), _ptr(ub),
+                                        _ptr(ud), m, C.byref(n)))
+        form = self.evaltype
+        out = []
+        for g in range(n.value):
+            shapes = tile_shapes(form, int(p[g]))
+            out.append(dict(p=int(p[g]), candidate=int(best[g]), epb=shapes[int(best[g])],
+                            us_per_launch=float(ub[g]), us_default=float(ud[g])))
+        return out

@@ -139,3 +139,23 @@ def test_serve_request_stream_ranges():
     assert np.all(gaps > 0) and abs(gaps.mean() - 0.25) < 0.02  # Poisson, rate 4/s
     frac = np.mean([r["tol_class"] for r in reqs])
     assert 0.45 < frac < 0.55
+
+
+def test_tile_shapes_host_side():
+    assert dgk.tile_shapes("HIP_MP", 7) == [1, 2, 4]
+    assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16]
+    assert dgk.tile_shapes("HIP_MP", 15) == [1, 2]
+    assert dgk.tile_shapes("HIP", 7) == [1]
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.tile_select("HIP", 7, 1)
+    assert ei.value.code == dgk.ERR_ARG
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.tile_select("HIP_MP", 15, 2)
+    assert ei.value.code == dgk.ERR_ARG
+    with pytest.raises(dgk.DgkError) as ei:
+        dgk.tile_shapes("GENERAL", 7)
+    assert ei.value.code == dgk.ERR_UNSUPPORTED
+    dgk.tile_select("HIP_MP", 7, 2)
+    assert dgk.tile_selected("HIP_MP", 7) == 2
+    dgk.tile_select("HIP_MP", 7, 0)
+    assert dgk.tile_selected("HIP_MP", 7) == 0

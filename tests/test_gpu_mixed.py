@@ -97,3 +97,54 @@ def test_fp64_stored_lift_kernel_for_non_separable_lift():
         s.set_fields(q, u, v, w)
         got = s.cal_tend()
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+def test_mp_tile_shapes_give_identical_results(p):
+    """Every tile shape (elements per workgroup) gives the same bits, also with
+    a partial last workgroup (30 elements)."""
+    ne = (3, 2, 5) if p <= 7 else (2, 1, 3)
+    q, u, v, w = inputs.random_fields(*ne, p, seed=700 + p, vel_scale=0.8)
+    dt = 0.2 * inputs.cfl_dt(inputs.lgl_nodes(p), ne, (0.8, 0.8, 0.8))
+    got = []
+    try:
+        for cand in range(len(dgk.tile_shapes("HIP_MP", p))):
+            dgk.tile_select("HIP_MP", p, cand)
+            with dgk.Solver.structured("HIP_MP", *ne, p) as s:
+                s.set_fields(q, u, v, w)
+                s.step(dt, 3)
+                got.append(s.get_field(dgk.FIELD_Q))
+    finally:
+        dgk.tile_select("HIP_MP", p, 0)
+    for k in range(1, len(got)):
+        assert np.array_equal(got[k], got[0]), k
+
+
+def test_autotune_solver_and_batch():
+    wl = inputs.c1_sine_noise()
+    with dgk.Solver.structured("HIP_MP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        s.step(wl.dt, 2)
+        before = s.get_field(dgk.FIELD_Q)
+        r = s.autotune(reps=3)
+        assert 0 <= r["best"] < len(dgk.tile_shapes("HIP_MP", wl.p))
+        assert len(r["us_per_launch"]) == len(dgk.tile_shapes("HIP_MP", wl.p))
+        assert all(t > 0 for t in r["us_per_launch"])
+        assert dgk.tile_selected("HIP_MP", wl.p) == r["best"]
+        assert np.array_equal(s.get_field(dgk.FIELD_Q), before)  # state untouched
+    ens = inputs.ensemble(n=6, seed=31, ne_choices=(2, 3), p_choices=(3, 7), nstep=4)
+    with dgk.Batch("HIP_MP", ens) as b, dgk.Batch("HIP_MP", ens) as ref:
+        for bb in (b, ref):
+            for k, it in enumerate(ens):
+                bb.init_gaussian(k, it["centre"], it["sigma"], it["vel"])
+        ref.step(2)
+        b.step(1)
+        rec = b.autotune(reps=3)
+        assert sorted(x["p"] for x in rec) == sorted({it["p"] for it in ens})
+        for x in rec:
+            assert x["epb"] in dgk.tile_shapes("HIP_MP", x["p"]) and x["us_per_launch"] > 0
+        b.step(1)  # re-captured with the tuned shapes
+        for k in range(len(ens)):
+            assert np.array_equal(b.get_field(k), ref.get_field(k)), k
+    for p in (3, 7, wl.p):
+        dgk.tile_select("HIP_MP", p, 0)
