@@ -214,7 +214,10 @@ def main():
     dof = r["dof"]
     bytes_per_launch = BYTES_PER_DOF_UPDATE[args.form] / 3.0 * dof
     achieved = bytes_per_launch / r["launch_s"] / 1e9
-    kname = "stage_f64<8>" if args.form == "HIP" else "stage_mp<8>"
+    if args.form == "HIP":
+        kname = "stage_f64<8>"
+    else:
+        kname = f"stage_mp<8, {dgk.tile_shapes('HIP_MP', 7)[dgk.tile_selected('HIP_MP', 7)]}>"
     line = {
         "metric": METRIC,
         "value": r["value"],
