@@ -208,11 +208,19 @@ int dgk_batch_synchronize(dgk_batch *b);
 int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
 
 /* ---- tile shapes and the autotuner --------------------------------------
- * The stage kernels come in one or more tile shapes per order: the number of
- * elements one workgroup advances (HIP_MP: one, two or four wavefronts of
- * whole elements for p <= 7, one or two elements above; HIP: one shape).
- * Candidate 0 is the default.  The selection holds for every later launch
- * of that form and order in the process.  dgk_autotune times each candidate
+ * The stage kernels come in several tile shapes per order, given as the
+ * number of elements one workgroup advances.
+ *   HIP_MP: the MFMA kernel with one, two or four wavefronts of whole
+ *           elements for p <= 7, one or two elements above (candidate 0 is
+ *           the default).
+ *   HIP:    candidate 0 is the VALU kernel; candidates 1.. are the FP64
+ *           MFMA kernel's shapes as for HIP_MP, limited to 160 KiB of LDS.
+ *           The default is the VALU kernel for p <= 7 and p = 10..12, and the
+ *           one-element MFMA shape for p = 8, 9, 13..15.
+ * dgk_tile_select(form, p, -1) restores the default.  The selection holds
+ * for every later launch of that form and order in the process, except that
+ * launches with a given flux (dgk_cal_dqdt) or a stored, non-separable Lift
+ * always use the VALU kernels.  dgk_autotune times each candidate
  * with HIP events over reps RK-stage launches on the solver's own mesh
  * (writing only the solver's scratch rotation buffer) and selects the
  * fastest; us_per_launch[c] receives candidate c's time.  dgk_batch_autotune
@@ -220,7 +228,9 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  * order, the chosen candidate, its time and the default's time (the batch's
  * captured step is re-captured at the next dgk_batch_step).  No counterpart
  * in the reference, whose OPT1 kernels are generated per order
- * (mod_dg_optr_kernel_opt1.F90.erb); results do not depend on the shape. */
+ * (mod_dg_optr_kernel_opt1.F90.erb).  The MFMA shapes of one form give the
+ * same bits as each other; the FP64 VALU and MFMA kernels differ by
+ * rounding only, both within the FP64 parity bar. */
 int dgk_tile_shapes(int evaltype, int p, int *epb, int max, int *n);
 int dgk_tile_select(int evaltype, int p, int cand);
 int dgk_tile_selected(int evaltype, int p, int *cand);

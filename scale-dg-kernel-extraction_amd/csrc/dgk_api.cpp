@@ -918,8 +918,8 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls) {
 namespace {
 
 int tile_shapes(int evaltype, int Nq, int *epb) {
-  if (evaltype == DGK_EVAL_HIP_MP) return stage_mp_shapes(Nq, epb, 3);
-  if (evaltype == DGK_EVAL_HIP) return stage_f64_shapes(Nq, epb, 3);
+  if (evaltype == DGK_EVAL_HIP_MP) return stage_mp_shapes(Nq, epb, kMaxTileShapes);
+  if (evaltype == DGK_EVAL_HIP) return stage_f64_shapes(Nq, epb, kMaxTileShapes);
   return 0;
 }
 
@@ -933,12 +933,11 @@ int check_form_order(int evaltype, int p) {
 
 int tile_select(int evaltype, int Nq, int cand) {
   if (evaltype == DGK_EVAL_HIP_MP) return stage_mp_select(Nq, cand);
-  if (cand != 0) return set_error(DGK_ERR_ARG, "the FP64 stage kernel has one tile shape");
-  return DGK_OK;
+  return stage_f64_select(Nq, cand);
 }
 
 int tile_selected(int evaltype, int Nq) {
-  return evaltype == DGK_EVAL_HIP_MP ? stage_mp_selected(Nq) : 0;
+  return evaltype == DGK_EVAL_HIP_MP ? stage_mp_selected(Nq) : stage_f64_selected(Nq);
 }
 
 // Average device time of one stage launch of s, in microseconds.
@@ -961,7 +960,7 @@ int time_stage(dgk_solver *s, int reps, double *us) {
 }
 
 int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
-  int epb[3];
+  int epb[kMaxTileShapes];
   const int nc = tile_shapes(s->evaltype, s->Nq, epb);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "no tile shapes for this form and order");
   int arg = 0;
@@ -990,7 +989,7 @@ int dgk_tile_shapes(int evaltype, int p, int *epb, int max, int *n) {
   clear_error();
   DGK_TRY(check_form_order(evaltype, p));
   if (!n || (max > 0 && !epb)) return set_error(DGK_ERR_ARG, "NULL argument");
-  int e[3];
+  int e[kMaxTileShapes];
   const int nc = tile_shapes(evaltype, p + 1, e);
   for (int c = 0; c < nc && c < max; ++c) epb[c] = e[c];
   *n = nc;
@@ -1025,9 +1024,9 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
   for (dgk_solver *s : b->groups) DGK_HIP(hipStreamSynchronize(s->stream));
   for (size_t g = 0; g < b->groups.size(); ++g) {
     dgk_solver *s = b->groups[g];
-    double t[3] = {0, 0, 0};
+    double t[kMaxTileShapes] = {0, 0, 0, 0};
     int arg = 0, nc = 0;
-    DGK_TRY(autotune(s, reps, &arg, t, 3, &nc));
+    DGK_TRY(autotune(s, reps, &arg, t, kMaxTileShapes, &nc));
     if ((int)g < max) {
       if (p) p[g] = b->group_p[g];
       if (best) best[g] = arg;

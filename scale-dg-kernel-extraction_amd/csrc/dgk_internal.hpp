@@ -141,7 +141,7 @@ int prepare_stage_f64(int Nq);
 int prepare_stage_mp(int Nq);
 
 // ---------------------------------------------------------------------------
-// Kernel parameter blocks and launchers (kernels_f64.hip, kernels_mp.hip).
+// Kernel parameter blocks and launchers (kernels_f64.hip, kernels_mfma.hip).
 // ---------------------------------------------------------------------------
 struct StageArgs {
   const double *q;        // stage input, owned part (Np*Ne)
@@ -192,8 +192,16 @@ int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
 int stage_mp_shapes(int Nq, int *epb, int n);
 int stage_mp_select(int Nq, int cand);
 int stage_mp_selected(int Nq);
-// The FP64 stage kernel has one tile shape per order (fills epb[0], returns 1).
+// FP64: candidate 0 is the VALU kernel, 1.. the FP64 MFMA kernel's shapes.
+constexpr int kMaxTileShapes = 4;
 int stage_f64_shapes(int Nq, int *epb, int n);
+int stage_f64_select(int Nq, int cand);
+int stage_f64_selected(int Nq);
+// FP64 MFMA stage kernel (kernels_mfma.hip): shapes, attributes, launch of
+// shape candidate `cand`.
+int tile_f64_shapes(int Nq, int *epb, int n);
+int prepare_tile_f64(int Nq);
+int launch_tile_f64(int Nq, int cand, const StageArgs &a, hipStream_t s);
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,
                             const double *v, const double *w,
                             const int32_t *nbr, const int32_t *vmapP,

@@ -570,8 +570,17 @@ unsigned grid1d(int64_t n, int threads) {
 
 }  // namespace
 
+// Default FP64 tile shape per Nq, from per-order timings on MI355X at about
+// 1e7 DOF: the VALU kernel up to p = 7 and at p = 10..12, the one-element
+// MFMA kernel at p = 8, 9 and 13..15.
+constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
+// Selected FP64 tile shape per Nq (see stage_f64_shapes).
+int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
+
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
+  if (Nq >= kMinNq && Nq <= kMaxNq && g_f64_shape[Nq] > 0 && !a.flux_in && a.ell)
+    return launch_tile_f64(Nq, g_f64_shape[Nq] - 1, a, s);
   switch (Nq) {
   case 2: return launch_nq<2>(a, s);
   case 3: return launch_nq<3>(a, s);
@@ -593,6 +602,7 @@ int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
 }
 
 int prepare_stage_f64(int Nq) {
+  DGK_TRY(prepare_tile_f64(Nq));
   switch (Nq) {
   case 2: return prepare_nq<2>();
   case 3: return prepare_nq<3>();
@@ -613,17 +623,38 @@ int prepare_stage_f64(int Nq) {
   }
 }
 
+// FP64 tile shapes: candidate 0 is the VALU kernel above (stage_f64), 1.. are
+// the FP64 MFMA kernel's shapes (kernels_mfma.hip).  The MFMA kernel computes
+// the flux itself and applies the separable Lift, so launches that pass a
+// flux (cal_dqdt) or need the stored Lift always take stage_f64 /
+// stage_f64_lift.
 int stage_f64_shapes(int Nq, int *epb, int n) {
-  if (!epb || n < 1) return 0;
+  if (!epb || n < kMaxTileShapes) return 0;
   switch (Nq) {
 #define DGK_F64_SHAPE(N) \
-  case N: epb[0] = Staged<N>::EPB; return 1;
+  case N: epb[0] = Staged<N>::EPB; break;
   DGK_F64_SHAPE(2) DGK_F64_SHAPE(3) DGK_F64_SHAPE(4) DGK_F64_SHAPE(5) DGK_F64_SHAPE(6)
   DGK_F64_SHAPE(7) DGK_F64_SHAPE(8) DGK_F64_SHAPE(9) DGK_F64_SHAPE(10) DGK_F64_SHAPE(11)
   DGK_F64_SHAPE(12) DGK_F64_SHAPE(13) DGK_F64_SHAPE(14) DGK_F64_SHAPE(15) DGK_F64_SHAPE(16)
 #undef DGK_F64_SHAPE
   default: return 0;
   }
+  return 1 + tile_f64_shapes(Nq, epb + 1, n - 1);
+}
+
+int stage_f64_select(int Nq, int cand) {
+  int e[kMaxTileShapes];
+  const int nc = stage_f64_shapes(Nq, e, kMaxTileShapes);
+  if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  if (cand == -1) cand = kF64Default[Nq];
+  if (cand < 0 || cand >= nc)
+    return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
+  g_f64_shape[Nq] = cand;
+  return DGK_OK;
+}
+
+int stage_f64_selected(int Nq) {
+  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_f64_shape[Nq];
 }
 
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,

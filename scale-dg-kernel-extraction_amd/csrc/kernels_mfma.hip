@@ -1,7 +1,10 @@
-// Mixed-precision stage kernel (gfx950): the 1D-operator contractions and the
-// Lift run in FP32 on MFMA with FP32 accumulation; the state q, the face-flux
-// jumps, the volume fluxes q*u_d and the RK stage combination stay FP64.
-// Velocities are read from FP32 copies.
+// MFMA stage kernels (gfx950), one template for two precisions:
+//  * mixed (HIP_MP): the 1D-operator contractions and the Lift run in FP32 on
+//    v_mfma_f32_16x16x4f32 with FP32 accumulation; the state q, the face-flux
+//    jumps, the volume fluxes q*u_d and the RK stage combination stay FP64;
+//    velocities are read from FP32 copies;
+//  * FP64 (HIP, tile candidates): the same data flow in FP64 throughout, on
+//    v_mfma_f64_16x16x4f64.
 //
 // For each direction d the sum-factorised derivative and that direction's two
 // Lift faces form one GEMM over the block's "pencils" r (node lines along d):
@@ -20,9 +23,9 @@
 //      own traces of the x/y faces and the neighbour traces straight into
 //      registers (coalesced along (i,j));
 //   2. FP64 upwind fluxes and FP64 volume fluxes E_d q u_d, each rounded once
-//      to FP32, are written into three augmented A arrays in LDS, one pencil
-//      per row of KPAD floats;
-//   3. v_mfma_f32_16x16x4f32 over 16-pencil tiles; K is permuted so that lane
+//      to the operand precision, are written into three augmented A arrays in
+//      LDS, one pencil per row of KPAD values;
+//   3. MFMA over 16-pencil tiles; K is permuted so that lane
 //      (col, kq) keeps rows l = kq*KSTEPS + ks of B in registers and reads A
 //      at fixed offsets from its own base; the result overwrites the tile's
 //      rows in place;
@@ -37,6 +40,30 @@ namespace dgk {
 namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Operand precision of the contraction GEMMs.
+// Both MFMAs take A(row = lane%16, k = lane/16) and B(k = lane/16,
+// col = lane%16); result value t of a lane lies in column lane%16 and row
+// RSTRIDE_Q*(lane/16) + RSTRIDE_T*t (checked on the device for both).
+struct PrecMixed {
+  using vel_t = float;   // velocity copies read by the kernel
+  using op_t = float;    // A, B operands in LDS / registers
+  using acc_t = f32x4;   // 16x16 result, 4 values per lane
+  static constexpr int RSTRIDE_Q = 4, RSTRIDE_T = 1;
+  static __device__ __forceinline__ acc_t mfma(op_t a, op_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+struct PrecF64 {
+  using vel_t = double;
+  using op_t = double;
+  using acc_t = f64x4;
+  static constexpr int RSTRIDE_Q = 1, RSTRIDE_T = 4;
+  static __device__ __forceinline__ acc_t mfma(op_t a, op_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
 
 __device__ __forceinline__ int xcd_block_mp(int b, int n) {
   const int per = n >> 3, rem = n & 7;
@@ -50,8 +77,8 @@ __device__ __forceinline__ int64_t uniform_mp(int64_t v) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Tile shape of the mixed-precision stage kernel: EPB elements per block.
-template <int NQ, int EPB_>
+// Tile shape of the MFMA stage kernel: EPB elements per block.
+template <class P, int NQ, int EPB_>
 struct MP {
   static constexpr int NQ2 = NQ * NQ;
   static constexpr int NP = NQ2 * NQ;
@@ -64,8 +91,8 @@ struct MP {
   static constexpr int KAUG = NQ + 2;
   static constexpr int KSTEPS = (KAUG + 3) / 4;
   static constexpr int KPAD = 4 * KSTEPS;
-  static constexpr int AREA = ROWS * KPAD;                  // floats per direction
-  static constexpr size_t LDS_BYTES = (size_t)3 * AREA * 4;
+  static constexpr int AREA = ROWS * KPAD;                  // operands per direction
+  static constexpr size_t LDS_BYTES = (size_t)3 * AREA * sizeof(typename P::op_t);
   static constexpr int JOBS = 3 * MTILES;                   // (direction, tile) GEMMs
   static constexpr int JPW = (JOBS + WAVES - 1) / WAVES;
 };
@@ -83,21 +110,32 @@ constexpr int mp_epb(int cand) {
   return cand == 0 ? 1 : 2;
 }
 
-template <int NQ, int EPB>
-__global__ void __launch_bounds__((MP<NQ, EPB>::THREADS))
-stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restrict__ u_,
-         const float *__restrict__ v_, const float *__restrict__ w_, double *out_,
+// Tile shapes whose LDS fits one CU (160 KiB).
+template <class P, int NQ>
+constexpr int mp_count() {
+  constexpr int n = NQ * NQ <= 64 ? 3 : 2;
+  return MP<P, NQ, mp_epb<NQ>(n - 1)>::LDS_BYTES <= 160 * 1024 ? n : n - 1;
+}
+
+template <class P, int NQ, int EPB>
+__global__ void __launch_bounds__((MP<P, NQ, EPB>::THREADS))
+stage_mfma(const double *__restrict__ q_, const double *q0_,
+         const typename P::vel_t *__restrict__ u_, const typename P::vel_t *__restrict__ v_,
+         const typename P::vel_t *__restrict__ w_, double *out_,
          const int32_t *__restrict__ nbr_, const int32_t *__restrict__ vmapP_,
-         const float *__restrict__ D_, const float *__restrict__ ell_,
+         const typename P::op_t *__restrict__ D_, const typename P::op_t *__restrict__ ell_,
          const double *__restrict__ geo_, const uint32_t *__restrict__ geoid_,
          const int32_t *__restrict__ order_, const int64_t Ne_, const double ca_,
          const double cb_, const double dt_, const int mode_, const int axis_aligned_,
          const int dt_geo_) {
-  using C = MP<NQ, EPB>;
+  using C = MP<P, NQ, EPB>;
+  using vel_t = typename P::vel_t;
+  using op_t = typename P::op_t;
   constexpr int NQ2 = C::NQ2, NP = C::NP, NFT = 6 * NQ2, KPAD = C::KPAD, KSTEPS = C::KSTEPS;
   constexpr int AREA = C::AREA;
-  extern __shared__ float smem_mp[];
-  float *const Ax = smem_mp, *const Ay = smem_mp + AREA, *const Az = smem_mp + 2 * AREA;
+  extern __shared__ double smem_mp[];
+  op_t *const As = (op_t *)smem_mp;
+  op_t *const Ax = As, *const Ay = As + AREA, *const Az = As + 2 * AREA;
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -126,9 +164,9 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
 
   // ---- 1. loads straight into registers ------------------------------------
   double qc[NQ];
-  float uc[NQ], vc[NQ], wc[NQ];
+  vel_t uc[NQ], vc[NQ], wc[NQ];
   double qM[4], qP[6];
-  float vM[4], vP[6];  // velocity component along the face axis
+  vel_t vM[4], vP[6];  // velocity component along the face axis
   if (active) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
@@ -154,7 +192,7 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
 
   // ---- 2. FP64 fluxes and volume fluxes -> augmented A rows (FP32) ---------
   if (active) {
-    float fl[6];
+    op_t fl[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       const double qm = f == 4 ? qc[0] : f == 5 ? qc[NQ - 1] : qM[f];
@@ -162,7 +200,7 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
       if (axal) {
         const int ax = kFaceAxis[f];
         const double na = g[kGeoNormal + 3 * f + ax];
-        const float vm = f == 4 ? wc[0] : f == 5 ? wc[NQ - 1] : vM[f];
+        const vel_t vm = f == 4 ? wc[0] : f == 5 ? wc[NQ - 1] : vM[f];
         VelM = (double)vm * na;
         VelP = (double)vP[f] * na;
       } else {
@@ -176,21 +214,21 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
         VelP = (double)u_[pidx] * n1 + (double)v_[pidx] * n2 + (double)w_[pidx] * n3;
       }
       const double alpha = 0.5 * fabs(VelP + VelM);
-      fl[f] = (float)(0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qm * VelM - alpha * (qP[f] - qm)));
+      fl[f] = (op_t)(0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qm * VelM - alpha * (qP[f] - qm)));
     }
     const double Ex = g[kGeoEscale], Ey = g[kGeoEscale + 1], Ez = g[kGeoEscale + 2];
     const int R = el * NQ2;
-    float *ax = Ax + (R + j) * KPAD + i;  // node (i,j,k): x pencil (j,k), position i
-    float *ay = Ay + (R + i) * KPAD + j;  //               y pencil (i,k), position j
-    float *az = Az + (R + ij) * KPAD;     //               z pencil (i,j), position k
+    op_t *ax = Ax + (R + j) * KPAD + i;  // node (i,j,k): x pencil (j,k), position i
+    op_t *ay = Ay + (R + i) * KPAD + j;  //               y pencil (i,k), position j
+    op_t *az = Az + (R + ij) * KPAD;     //               z pencil (i,j), position k
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      ax[k * NQ * KPAD] = (float)(Ex * (qc[k] * (double)uc[k]));
-      ay[k * NQ * KPAD] = (float)(Ey * (qc[k] * (double)vc[k]));
-      az[k] = (float)(Ez * (qc[k] * (double)wc[k]));
+      ax[k * NQ * KPAD] = (op_t)(Ex * (qc[k] * (double)uc[k]));
+      ay[k * NQ * KPAD] = (op_t)(Ey * (qc[k] * (double)vc[k]));
+      az[k] = (op_t)(Ez * (qc[k] * (double)wc[k]));
     }
     // face node (i,j) of the faces normal to d is pencil ij of direction d
-    float *rx = Ax + (R + ij) * KPAD, *ry = Ay + (R + ij) * KPAD;
+    op_t *rx = Ax + (R + ij) * KPAD, *ry = Ay + (R + ij) * KPAD;
     rx[NQ] = fl[3];      // -x
     rx[NQ + 1] = fl[1];  // +x
     ry[NQ] = fl[0];      // -y
@@ -199,9 +237,9 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
     az[NQ + 1] = fl[5];  // +z
 #pragma unroll
     for (int l = NQ + 2; l < KPAD; ++l) {
-      rx[l] = 0.f;
-      ry[l] = 0.f;
-      az[l] = 0.f;
+      rx[l] = (op_t)0;
+      ry[l] = (op_t)0;
+      az[l] = (op_t)0;
     }
   }
   // RK base column, in flight during the GEMMs
@@ -217,28 +255,27 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
   // row depends only on its own A row, and those rows are never read back.
   {
     const int col = lane & 15, kq = lane >> 4;
-    float bop[KSTEPS];
+    op_t bop[KSTEPS];
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
       const int l = kq * KSTEPS + ks;
-      bop[ks] = col >= NQ ? 0.f : l < NQ ? D_[col + l * NQ] : l == NQ ? ell_[col]
-                : l == NQ + 1 ? ell_[NQ + col] : 0.f;
+      bop[ks] = col >= NQ ? (op_t)0 : l < NQ ? D_[col + l * NQ] : l == NQ ? ell_[col]
+                : l == NQ + 1 ? ell_[NQ + col] : (op_t)0;
     }
     const int rd = col * KPAD + kq * KSTEPS;  // A read offset of this lane in a tile
-    const int wr = 4 * kq * KPAD + col;       // result offset (tile rows 4kq + t4)
+    const int wr = P::RSTRIDE_Q * kq * KPAD + col;  // result offset of value 0
 #pragma unroll
     for (int jj = 0; jj < C::JPW; ++jj) {
       const int job = wave + jj * C::WAVES;
       if (job >= C::JOBS) break;  // wave-uniform
       const int d = job / C::MTILES, t = job - d * C::MTILES;
-      float *tile = smem_mp + d * AREA + t * 16 * KPAD;
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      op_t *tile = As + d * AREA + t * 16 * KPAD;
+      typename P::acc_t acc = {0, 0, 0, 0};
 #pragma unroll
-      for (int ks = 0; ks < KSTEPS; ++ks)
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(tile[rd + ks], bop[ks], acc, 0, 0, 0);
+      for (int ks = 0; ks < KSTEPS; ++ks) acc = P::mfma(tile[rd + ks], bop[ks], acc);
       if (col < NQ) {
 #pragma unroll
-        for (int t4 = 0; t4 < 4; ++t4) tile[wr + t4 * KPAD] = acc[t4];
+        for (int t4 = 0; t4 < 4; ++t4) tile[wr + t4 * P::RSTRIDE_T * KPAD] = acc[t4];
       }
     }
   }
@@ -247,9 +284,9 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
   // ---- 4. FP64 combine and RK epilogue -------------------------------------
   if (active) {
     const int R = el * NQ2;
-    const float *ax = Ax + (R + j) * KPAD + i;
-    const float *ay = Ay + (R + i) * KPAD + j;
-    const float *az = Az + (R + ij) * KPAD;
+    const op_t *ax = Ax + (R + j) * KPAD + i;
+    const op_t *ay = Ay + (R + i) * KPAD + j;
+    const op_t *az = Az + (R + ij) * KPAD;
     const double dtv = dt_geo_ ? g[kGeoDt] : dt_;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
@@ -266,27 +303,27 @@ stage_mp(const double *__restrict__ q_, const double *q0_, const float *__restri
   }
 }
 
-template <int NQ, int EPB>
-int prepare_mp_shape() {
+template <class P, int NQ, int EPB>
+int prepare_shape() {
   static int prepared_dev = -1;
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_mp<NQ, EPB>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)MP<NQ, EPB>::LDS_BYTES));
+                                (int)MP<P, NQ, EPB>::LDS_BYTES));
     prepared_dev = dev;
   }
   return DGK_OK;
 }
 
-template <int NQ, int EPB>
-int launch_mp_shape(const StageArgsMP &a, hipStream_t s) {
-  using T = MP<NQ, EPB>;
+template <class P, int NQ, int EPB, class Args>
+int launch_shape(const Args &a, hipStream_t s) {
+  using T = MP<P, NQ, EPB>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  DGK_TRY((prepare_mp_shape<NQ, EPB>()));
-  hipLaunchKernelGGL((stage_mp<NQ, EPB>), dim3((unsigned)nblk), dim3(T::THREADS),
+  DGK_TRY((prepare_shape<P, NQ, EPB>()));
+  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB>), dim3((unsigned)nblk), dim3(T::THREADS),
                      T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr, a.vmapP, a.D,
                      a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode,
                      a.axis_aligned, a.dt_geo);
@@ -294,87 +331,92 @@ int launch_mp_shape(const StageArgsMP &a, hipStream_t s) {
   return DGK_OK;
 }
 
-// Tile-shape candidate selected per Nq (index into mp_epb).
+// Tile-shape candidate selected per Nq (index into mp_epb) for the mixed form.
 int g_mp_shape[kMaxNq + 1] = {0};
 
-template <int NQ>
-int prepare_mp_nq() {
-  DGK_TRY((prepare_mp_shape<NQ, mp_epb<NQ>(0)>()));
-  DGK_TRY((prepare_mp_shape<NQ, mp_epb<NQ>(1)>()));
-  DGK_TRY((prepare_mp_shape<NQ, mp_epb<NQ>(2)>()));
+template <class P, int NQ>
+int prepare_nq_all() {
+  DGK_TRY((prepare_shape<P, NQ, mp_epb<NQ>(0)>()));
+  if constexpr (mp_count<P, NQ>() > 1) DGK_TRY((prepare_shape<P, NQ, mp_epb<NQ>(1)>()));
+  if constexpr (mp_count<P, NQ>() > 2) DGK_TRY((prepare_shape<P, NQ, mp_epb<NQ>(2)>()));
   return DGK_OK;
 }
 
-template <int NQ>
-int launch_mp_nq(const StageArgsMP &a, hipStream_t s) {
-  switch (g_mp_shape[NQ]) {
-  case 1: return launch_mp_shape<NQ, mp_epb<NQ>(1)>(a, s);
-  case 2: return launch_mp_shape<NQ, mp_epb<NQ>(2)>(a, s);
-  default: return launch_mp_shape<NQ, mp_epb<NQ>(0)>(a, s);
-  }
+template <class P, int NQ, class Args>
+int launch_nq_cand(int cand, const Args &a, hipStream_t s) {
+  if constexpr (mp_count<P, NQ>() > 1)
+    if (cand == 1) return launch_shape<P, NQ, mp_epb<NQ>(1)>(a, s);
+  if constexpr (mp_count<P, NQ>() > 2)
+    if (cand == 2) return launch_shape<P, NQ, mp_epb<NQ>(2)>(a, s);
+  return launch_shape<P, NQ, mp_epb<NQ>(0)>(a, s);
 }
 
 }  // namespace
 
-int prepare_stage_mp(int Nq) {
-  switch (Nq) {
-  case 2: return prepare_mp_nq<2>();
-  case 3: return prepare_mp_nq<3>();
-  case 4: return prepare_mp_nq<4>();
-  case 5: return prepare_mp_nq<5>();
-  case 6: return prepare_mp_nq<6>();
-  case 7: return prepare_mp_nq<7>();
-  case 8: return prepare_mp_nq<8>();
-  case 9: return prepare_mp_nq<9>();
-  case 10: return prepare_mp_nq<10>();
-  case 11: return prepare_mp_nq<11>();
-  case 12: return prepare_mp_nq<12>();
-  case 13: return prepare_mp_nq<13>();
-  case 14: return prepare_mp_nq<14>();
-  case 15: return prepare_mp_nq<15>();
-  case 16: return prepare_mp_nq<16>();
-  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+#define DGK_NQ_SWITCH(NQV, CALL)                                                         \
+  switch (NQV) {                                                                         \
+  case 2: return CALL(2); case 3: return CALL(3); case 4: return CALL(4);                \
+  case 5: return CALL(5); case 6: return CALL(6); case 7: return CALL(7);                \
+  case 8: return CALL(8); case 9: return CALL(9); case 10: return CALL(10);              \
+  case 11: return CALL(11); case 12: return CALL(12); case 13: return CALL(13);          \
+  case 14: return CALL(14); case 15: return CALL(15); case 16: return CALL(16);          \
+  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");                   \
   }
+
+int prepare_stage_mp(int Nq) {
+#define DGK_CALL(N) prepare_nq_all<PrecMixed, N>()
+  DGK_NQ_SWITCH(Nq, DGK_CALL)
+#undef DGK_CALL
 }
 
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
-  switch (Nq) {
-  case 2: return launch_mp_nq<2>(a, s);
-  case 3: return launch_mp_nq<3>(a, s);
-  case 4: return launch_mp_nq<4>(a, s);
-  case 5: return launch_mp_nq<5>(a, s);
-  case 6: return launch_mp_nq<6>(a, s);
-  case 7: return launch_mp_nq<7>(a, s);
-  case 8: return launch_mp_nq<8>(a, s);
-  case 9: return launch_mp_nq<9>(a, s);
-  case 10: return launch_mp_nq<10>(a, s);
-  case 11: return launch_mp_nq<11>(a, s);
-  case 12: return launch_mp_nq<12>(a, s);
-  case 13: return launch_mp_nq<13>(a, s);
-  case 14: return launch_mp_nq<14>(a, s);
-  case 15: return launch_mp_nq<15>(a, s);
-  case 16: return launch_mp_nq<16>(a, s);
-  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
-  }
+#define DGK_CALL(N) launch_nq_cand<PrecMixed, N>(g_mp_shape[N], a, s)
+  DGK_NQ_SWITCH(Nq, DGK_CALL)
+#undef DGK_CALL
 }
+
+int prepare_tile_f64(int Nq) {
+#define DGK_CALL(N) prepare_nq_all<PrecF64, N>()
+  DGK_NQ_SWITCH(Nq, DGK_CALL)
+#undef DGK_CALL
+}
+
+int launch_tile_f64(int Nq, int cand, const StageArgs &a, hipStream_t s) {
+  if (a.Ne < 1) return DGK_OK;
+#define DGK_CALL(N) launch_nq_cand<PrecF64, N>(cand, a, s)
+  DGK_NQ_SWITCH(Nq, DGK_CALL)
+#undef DGK_CALL
+}
+
+namespace {
+template <class P, int NQ>
+int shapes_of(int *epb) {
+  const int n = mp_count<P, NQ>();
+  for (int c = 0; c < n; ++c) epb[c] = mp_epb<NQ>(c);
+  return n;
+}
+}  // namespace
 
 int stage_mp_shapes(int Nq, int *epb, int n) {
   if (Nq < kMinNq || Nq > kMaxNq || !epb || n < 3) return 0;
-  switch (Nq) {
-#define DGK_MP_SHAPES(N) \
-  case N: epb[0] = mp_epb<N>(0); epb[1] = mp_epb<N>(1); epb[2] = mp_epb<N>(2); break;
-  DGK_MP_SHAPES(2) DGK_MP_SHAPES(3) DGK_MP_SHAPES(4) DGK_MP_SHAPES(5) DGK_MP_SHAPES(6)
-  DGK_MP_SHAPES(7) DGK_MP_SHAPES(8) DGK_MP_SHAPES(9) DGK_MP_SHAPES(10) DGK_MP_SHAPES(11)
-  DGK_MP_SHAPES(12) DGK_MP_SHAPES(13) DGK_MP_SHAPES(14) DGK_MP_SHAPES(15) DGK_MP_SHAPES(16)
-#undef DGK_MP_SHAPES
-  }
-  return Nq * Nq <= 64 ? 3 : 2;
+#define DGK_CALL(N) shapes_of<PrecMixed, N>(epb)
+  DGK_NQ_SWITCH(Nq, DGK_CALL)
+#undef DGK_CALL
+}
+
+int tile_f64_shapes(int Nq, int *epb, int n) {
+  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < 3) return 0;
+#define DGK_CALL(N) shapes_of<PrecF64, N>(epb)
+  DGK_NQ_SWITCH(Nq, DGK_CALL)
+#undef DGK_CALL
 }
 
 int stage_mp_select(int Nq, int cand) {
-  if (Nq < kMinNq || Nq > kMaxNq) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
-  const int nc = Nq * Nq <= 64 ? 3 : 2;
+  int e[3];
+  const int nc = stage_mp_shapes(Nq, e, 3);
+  if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  if (cand == -1) cand = 0;  // the default
   if (cand < 0 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
   g_mp_shape[Nq] = cand;
@@ -384,5 +426,7 @@ int stage_mp_select(int Nq, int cand) {
 int stage_mp_selected(int Nq) {
   return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_mp_shape[Nq];
 }
+
+#undef DGK_NQ_SWITCH
 
 }  // namespace dgk

@@ -147,8 +147,8 @@ def tile_shapes(evaltype, p: int) -> list:
     """Elements per workgroup of each tile-shape candidate of a stage kernel."""
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
-    e, n = np.zeros(3, np.int32), C.c_int(0)
-    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 3, C.byref(n)))
+    e, n = np.zeros(4, np.int32), C.c_int(0)
+    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 4, C.byref(n)))
     return e[:n.value].tolist()
 
 
@@ -346,8 +346,8 @@ class Solver:
         """Time every tile shape of this solver's form and order on its mesh and
         select the fastest (process-wide for that form and order)."""
         best, n = C.c_int(0), C.c_int(0)
-        us = np.zeros(3)
-        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 3, C.byref(n)))
+        us = np.zeros(4)
+        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 4, C.byref(n)))
         return dict(best=best.value, us_per_launch=us[:n.value].tolist())
 
 

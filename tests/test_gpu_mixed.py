@@ -115,7 +115,7 @@ def test_mp_tile_shapes_give_identical_results(p):
                 s.step(dt, 3)
                 got.append(s.get_field(dgk.FIELD_Q))
     finally:
-        dgk.tile_select("HIP_MP", p, 0)
+        dgk.tile_select("HIP_MP", p, -1)
     for k in range(1, len(got)):
         assert np.array_equal(got[k], got[0]), k
 
@@ -147,4 +147,4 @@ def test_autotune_solver_and_batch():
         for k in range(len(ens)):
             assert np.array_equal(b.get_field(k), ref.get_field(k)), k
     for p in (3, 7, wl.p):
-        dgk.tile_select("HIP_MP", p, 0)
+        dgk.tile_select("HIP_MP", p, -1)

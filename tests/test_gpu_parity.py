@@ -180,3 +180,34 @@ def test_c2_full_size_short_run(nstep):
         s.step(wl.dt, nstep)
         got = s.get_field(dgk.FIELD_Q)
     assert maxrel(got, ref) <= LOOP_TOL
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+def test_fp64_tile_shapes_against_oracle(p):
+    """Every FP64 tile shape (candidate 0: VALU kernel; 1..: FP64 MFMA kernel
+    with 1, 2 or 4 wavefronts of elements) holds the tendency and loop bars
+    against the oracle; the MFMA shapes give the same bits as each other,
+    also with a partial last workgroup (30 elements)."""
+    ne = (3, 2, 5) if p <= 7 else (2, 1, 3)
+    m = oracle.Mesh(*ne, p)
+    q, u, v, w = inputs.random_fields(*ne, p, seed=900 + p, vel_scale=0.8)
+    dt = 0.2 * inputs.cfl_dt(inputs.lgl_nodes(p), ne, (0.8, 0.8, 0.8))
+    tend_ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    loop_ref, _, _ = m.run(q, u, v, w, dt, 3)
+    shapes = dgk.tile_shapes("HIP", p)
+    assert len(shapes) >= 2
+    loops = []
+    try:
+        for cand in range(len(shapes)):
+            dgk.tile_select("HIP", p, cand)
+            with dgk.Solver.structured("HIP", *ne, p) as s:
+                s.set_fields(q, u, v, w)
+                assert maxrel(s.cal_tend(), tend_ref) <= TEND_TOL, cand
+                s.step(dt, 3)
+                got = s.get_field(dgk.FIELD_Q)
+            assert maxrel(got, loop_ref) <= LOOP_TOL, cand
+            loops.append(got)
+    finally:
+        dgk.tile_select("HIP", p, -1)
+    for k in range(2, len(loops)):
+        assert np.array_equal(loops[k], loops[1]), k
