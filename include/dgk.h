@@ -211,12 +211,14 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  * The stage kernels come in several tile shapes per order, given as the
  * number of elements one workgroup advances.
  *   HIP_MP: the MFMA kernel with one, two or four wavefronts of whole
- *           elements for p <= 7, one or two elements above (candidate 0 is
- *           the default).
+ *           elements for p <= 7; for p >= 8 one element, two elements, and
+ *           one element processed direction by direction (a third of the
+ *           LDS).  Candidate 0 is the default.
  *   HIP:    candidate 0 is the VALU kernel; candidates 1.. are the FP64
  *           MFMA kernel's shapes as for HIP_MP, limited to 160 KiB of LDS.
- *           The default is the VALU kernel for p <= 7 and p = 10..12, and the
- *           one-element MFMA shape for p = 8, 9, 13..15.
+ *           The default is the VALU kernel for p <= 7 and p = 10, 11, and the
+ *           direction-by-direction MFMA shape (the last candidate) for p = 8,
+ *           9 and 12..15.
  * dgk_tile_select(form, p, -1) restores the default.  The selection holds
  * for every later launch of that form and order in the process, except that
  * launches with a given flux (dgk_cal_dqdt) or a stored, non-separable Lift

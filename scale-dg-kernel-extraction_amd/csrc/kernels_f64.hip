@@ -571,11 +571,12 @@ unsigned grid1d(int64_t n, int threads) {
 }  // namespace
 
 // Default FP64 tile shape per Nq, from per-order timings on MI355X at about
-// 1e7 DOF: the VALU kernel up to p = 7 and at p = 10..12, the one-element
-// MFMA kernel at p = 8, 9 and 13..15.
-constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
+// 1e7 DOF: the VALU kernel up to p = 7 and at p = 10, 11; the
+// direction-sequential one-element MFMA kernel (the last candidate) at p = 8,
+// 9 and 12..15.
+constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2};
 // Selected FP64 tile shape per Nq (see stage_f64_shapes).
-int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 1, 1};
+int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2};
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;

@@ -19,18 +19,22 @@
 // (tensorprod_divlike_dirXYZ, mod_dg_optr_kernel.f90:197-258).
 //
 // Data flow of one block (EPB elements, one lane per node column (i,j)):
-//   1. the lane loads its q column (FP64), its u, v, w columns (FP32), the
-//      own traces of the x/y faces and the neighbour traces straight into
+//   1. the lane loads its q column (FP64), its velocity columns, the own
+//      traces of the x/y faces and the neighbour traces straight into
 //      registers (coalesced along (i,j));
 //   2. FP64 upwind fluxes and FP64 volume fluxes E_d q u_d, each rounded once
-//      to the operand precision, are written into three augmented A arrays in
-//      LDS, one pencil per row of KPAD values;
-//   3. MFMA over 16-pencil tiles; K is permuted so that lane
-//      (col, kq) keeps rows l = kq*KSTEPS + ks of B in registers and reads A
-//      at fixed offsets from its own base; the result overwrites the tile's
-//      rows in place;
+//      to the operand precision, are written into augmented A arrays in LDS,
+//      one pencil per row of KPAD values;
+//   3. MFMA over 16-pencil tiles; K is permuted so that lane (col, kq) keeps
+//      rows l = kq*KSTEPS + ks of B in registers and reads A at fixed offsets
+//      from its own base; the result overwrites the tile's rows in place;
 //   4. the three direction results are summed in FP64, dqdt = -((X + Y) + Z),
 //      and RK-combined in FP64 (main.f90:69-73).
+// The direction-sequential variant (SEQ) runs steps 2-3 one direction at a
+// time through one LDS area, loading each velocity column only when its
+// direction comes up: a third of the LDS and fewer live registers, for large
+// orders; it adds the directions in the same order, so its results are the
+// same bits.
 #include <algorithm>
 
 #include "dgk_internal.hpp"
@@ -77,12 +81,14 @@ __device__ __forceinline__ int64_t uniform_mp(int64_t v) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Tile shape of the MFMA stage kernel: EPB elements per block.
-template <class P, int NQ, int EPB_>
+// Tile shape of the MFMA stage kernel: EPB elements per block, all three
+// directions at once or one after the other (SEQ).
+template <class P, int NQ, int EPB_, bool SEQ_>
 struct MP {
   static constexpr int NQ2 = NQ * NQ;
   static constexpr int NP = NQ2 * NQ;
   static constexpr int EPB = EPB_;
+  static constexpr bool SEQ = SEQ_;
   static constexpr int LANES = EPB * NQ2;                   // node columns per block
   static constexpr int THREADS = ((LANES + 63) / 64) * 64;
   static constexpr int WAVES = THREADS / 64;
@@ -92,50 +98,86 @@ struct MP {
   static constexpr int KSTEPS = (KAUG + 3) / 4;
   static constexpr int KPAD = 4 * KSTEPS;
   static constexpr int AREA = ROWS * KPAD;                  // operands per direction
-  static constexpr size_t LDS_BYTES = (size_t)3 * AREA * sizeof(typename P::op_t);
-  static constexpr int JOBS = 3 * MTILES;                   // (direction, tile) GEMMs
+  static constexpr int NAREA = SEQ ? 1 : 3;
+  static constexpr size_t LDS_BYTES = (size_t)NAREA * AREA * sizeof(typename P::op_t);
+  static constexpr int JOBS = NAREA * MTILES;               // (direction, tile) GEMMs per pass
   static constexpr int JPW = (JOBS + WAVES - 1) / WAVES;
 };
 
-// Candidate tile shapes per order (index 0 is the default): one, two or four
-// wavefronts of whole elements for Nq^2 <= 64, one or two elements above.
+// Candidate kinds per order: one, two or four wavefronts of whole elements
+// for Nq^2 <= 64; above, one element, two elements, and one element
+// direction by direction.  Kinds whose LDS exceeds a CU's 160 KiB are
+// dropped; the remaining ones are the candidates 0, 1, ... (0 is the
+// default).
 template <int NQ>
-constexpr int mp_epb(int cand) {
+constexpr int kind_epb(int k) {
   constexpr int NQ2 = NQ * NQ;
   if (NQ2 <= 64) {
-    const int waves = cand == 0 ? 1 : cand == 1 ? 2 : 4;
+    const int waves = k == 0 ? 1 : k == 1 ? 2 : 4;
     const int e = 64 * waves / NQ2;
     return e < 1 ? 1 : e;
   }
-  return cand == 0 ? 1 : 2;
+  return k == 1 ? 2 : 1;
 }
-
-// Tile shapes whose LDS fits one CU (160 KiB).
+template <int NQ>
+constexpr bool kind_seq(int k) {
+  return NQ * NQ > 64 && k == 2;
+}
 template <class P, int NQ>
-constexpr int mp_count() {
-  constexpr int n = NQ * NQ <= 64 ? 3 : 2;
-  return MP<P, NQ, mp_epb<NQ>(n - 1)>::LDS_BYTES <= 160 * 1024 ? n : n - 1;
+constexpr bool kind_ok(int k) {
+  return k == 0   ? MP<P, NQ, kind_epb<NQ>(0), kind_seq<NQ>(0)>::LDS_BYTES <= 160 * 1024
+         : k == 1 ? MP<P, NQ, kind_epb<NQ>(1), kind_seq<NQ>(1)>::LDS_BYTES <= 160 * 1024
+                  : MP<P, NQ, kind_epb<NQ>(2), kind_seq<NQ>(2)>::LDS_BYTES <= 160 * 1024;
+}
+template <class P, int NQ>
+constexpr int ncand() {
+  return (int)kind_ok<P, NQ>(0) + (int)kind_ok<P, NQ>(1) + (int)kind_ok<P, NQ>(2);
+}
+// kind of candidate c (-1 if none)
+template <class P, int NQ>
+constexpr int kind_of(int c) {
+  int n = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (!kind_ok<P, NQ>(k)) continue;
+    if (n == c) return k;
+    ++n;
+  }
+  return -1;
 }
 
-template <class P, int NQ, int EPB>
-__global__ void __launch_bounds__((MP<P, NQ, EPB>::THREADS))
+// One 16-pencil tile: A rows from `tile`, B in registers; the result
+// overwrites the tile's first NQ columns.
+template <class P, int KSTEPS, int KPAD, int NQ>
+__device__ __forceinline__ void mfma_tile(typename P::op_t *tile,
+                                          const typename P::op_t (&bop)[KSTEPS], int rd,
+                                          int wr, int col) {
+  typename P::acc_t acc = {0, 0, 0, 0};
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ++ks) acc = P::mfma(tile[rd + ks], bop[ks], acc);
+  if (col < NQ) {
+#pragma unroll
+    for (int t4 = 0; t4 < 4; ++t4) tile[wr + t4 * P::RSTRIDE_T * KPAD] = acc[t4];
+  }
+}
+
+template <class P, int NQ, int EPB, bool SEQ>
+__global__ void __launch_bounds__((MP<P, NQ, EPB, SEQ>::THREADS))
 stage_mfma(const double *__restrict__ q_, const double *q0_,
-         const typename P::vel_t *__restrict__ u_, const typename P::vel_t *__restrict__ v_,
-         const typename P::vel_t *__restrict__ w_, double *out_,
-         const int32_t *__restrict__ nbr_, const int32_t *__restrict__ vmapP_,
-         const typename P::op_t *__restrict__ D_, const typename P::op_t *__restrict__ ell_,
-         const double *__restrict__ geo_, const uint32_t *__restrict__ geoid_,
-         const int32_t *__restrict__ order_, const int64_t Ne_, const double ca_,
-         const double cb_, const double dt_, const int mode_, const int axis_aligned_,
-         const int dt_geo_) {
-  using C = MP<P, NQ, EPB>;
+           const typename P::vel_t *__restrict__ u_, const typename P::vel_t *__restrict__ v_,
+           const typename P::vel_t *__restrict__ w_, double *out_,
+           const int32_t *__restrict__ nbr_, const int32_t *__restrict__ vmapP_,
+           const typename P::op_t *__restrict__ D_, const typename P::op_t *__restrict__ ell_,
+           const double *__restrict__ geo_, const uint32_t *__restrict__ geoid_,
+           const int32_t *__restrict__ order_, const int64_t Ne_, const double ca_,
+           const double cb_, const double dt_, const int mode_, const int axis_aligned_,
+           const int dt_geo_) {
+  using C = MP<P, NQ, EPB, SEQ>;
   using vel_t = typename P::vel_t;
   using op_t = typename P::op_t;
   constexpr int NQ2 = C::NQ2, NP = C::NP, NFT = 6 * NQ2, KPAD = C::KPAD, KSTEPS = C::KSTEPS;
   constexpr int AREA = C::AREA;
   extern __shared__ double smem_mp[];
   op_t *const As = (op_t *)smem_mp;
-  op_t *const Ax = As, *const Ay = As + AREA, *const Az = As + 2 * AREA;
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -161,19 +203,27 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   const bool rk = mode_ == 1;
   const bool use_q0 = rk && ca_ != 0.0;
   const bool axal = axis_aligned_ != 0;
+  const int R = el * NQ2;
+  const vel_t *const velp[3] = {u_, v_, w_};
 
   // ---- 1. loads straight into registers ------------------------------------
   double qc[NQ];
-  vel_t uc[NQ], vc[NQ], wc[NQ];
+  vel_t uc[SEQ ? 1 : NQ], vc[SEQ ? 1 : NQ], wc[NQ];  // SEQ: w only at k = 0, NQ-1
   double qM[4], qP[6];
   vel_t vM[4], vP[6];  // velocity component along the face axis
   if (active) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       qc[k] = q_[base + k * NQ2];
-      uc[k] = u_[base + k * NQ2];
-      vc[k] = v_[base + k * NQ2];
-      wc[k] = w_[base + k * NQ2];
+      if constexpr (!SEQ) {
+        uc[k] = u_[base + k * NQ2];
+        vc[k] = v_[base + k * NQ2];
+        wc[k] = w_[base + k * NQ2];
+      }
+    }
+    if constexpr (SEQ) {
+      wc[0] = w_[base];
+      wc[NQ - 1] = w_[base + (NQ - 1) * NQ2];
     }
 #pragma unroll
     for (int f = 0; f < 4; ++f) {  // own traces of the x and y faces
@@ -190,9 +240,9 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
     }
   }
 
-  // ---- 2. FP64 fluxes and volume fluxes -> augmented A rows (FP32) ---------
+  // ---- 2a. FP64 upwind fluxes -------------------------------------------------
+  op_t fl[6];
   if (active) {
-    op_t fl[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       const double qm = f == 4 ? qc[0] : f == 5 ? qc[NQ - 1] : qM[f];
@@ -216,114 +266,152 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       const double alpha = 0.5 * fabs(VelP + VelM);
       fl[f] = (op_t)(0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qm * VelM - alpha * (qP[f] - qm)));
     }
-    const double Ex = g[kGeoEscale], Ey = g[kGeoEscale + 1], Ez = g[kGeoEscale + 2];
-    const int R = el * NQ2;
-    op_t *ax = Ax + (R + j) * KPAD + i;  // node (i,j,k): x pencil (j,k), position i
-    op_t *ay = Ay + (R + i) * KPAD + j;  //               y pencil (i,k), position j
-    op_t *az = Az + (R + ij) * KPAD;     //               z pencil (i,j), position k
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      ax[k * NQ * KPAD] = (op_t)(Ex * (qc[k] * (double)uc[k]));
-      ay[k * NQ * KPAD] = (op_t)(Ey * (qc[k] * (double)vc[k]));
-      az[k] = (op_t)(Ez * (qc[k] * (double)wc[k]));
-    }
-    // face node (i,j) of the faces normal to d is pencil ij of direction d
-    op_t *rx = Ax + (R + ij) * KPAD, *ry = Ay + (R + ij) * KPAD;
-    rx[NQ] = fl[3];      // -x
-    rx[NQ + 1] = fl[1];  // +x
-    ry[NQ] = fl[0];      // -y
-    ry[NQ + 1] = fl[2];  // +y
-    az[NQ] = fl[4];      // -z
-    az[NQ + 1] = fl[5];  // +z
-#pragma unroll
-    for (int l = NQ + 2; l < KPAD; ++l) {
-      rx[l] = (op_t)0;
-      ry[l] = (op_t)0;
-      az[l] = (op_t)0;
-    }
   }
-  // RK base column, in flight during the GEMMs
-  double q0v[NQ];
-  if (active && use_q0) {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) q0v[k] = q0_[base + k * NQ2];
-  }
-  __syncthreads();
 
-  // ---- 3. MFMA GEMMs over (direction, 16-pencil tile) jobs -----------------
-  // Rows of A beyond the block's active pencils hold stale LDS; each output
-  // row depends only on its own A row, and those rows are never read back.
-  {
-    const int col = lane & 15, kq = lane >> 4;
-    op_t bop[KSTEPS];
+  // B = [D^T ; l- ; l+ ; 0] rows of this lane (k permuted, see above)
+  const int col = lane & 15, kq = lane >> 4;
+  op_t bop[KSTEPS];
 #pragma unroll
-    for (int ks = 0; ks < KSTEPS; ++ks) {
-      const int l = kq * KSTEPS + ks;
-      bop[ks] = col >= NQ ? (op_t)0 : l < NQ ? D_[col + l * NQ] : l == NQ ? ell_[col]
-                : l == NQ + 1 ? ell_[NQ + col] : (op_t)0;
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int l = kq * KSTEPS + ks;
+    bop[ks] = col >= NQ ? (op_t)0 : l < NQ ? D_[col + l * NQ] : l == NQ ? ell_[col]
+              : l == NQ + 1 ? ell_[NQ + col] : (op_t)0;
+  }
+  const int rd = col * KPAD + kq * KSTEPS;        // A read offset of this lane in a tile
+  const int wr = P::RSTRIDE_Q * kq * KPAD + col;  // result offset of value 0
+  // Node (i,j,k) sits in the x pencil (j,k) at position i, in the y pencil
+  // (i,k) at position j and in the z pencil (i,j) at position k; face node
+  // (i,j) of the faces normal to d is pencil ij of direction d.
+  const int pos_off[3] = {(R + j) * KPAD + i, (R + i) * KPAD + j, (R + ij) * KPAD};
+  const int pos_step[3] = {NQ * KPAD, NQ * KPAD, 1};
+  const int face_m[3] = {3, 0, 4}, face_p[3] = {1, 2, 5};  // -d and +d faces
+
+  double q0v[NQ];
+  double tend[NQ];
+  if constexpr (!SEQ) {
+    // ---- 2b. volume fluxes of all three directions -> A arrays ------------
+    if (active) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double Ed = g[kGeoEscale + d];
+        op_t *a = As + d * AREA + pos_off[d];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+          const vel_t vel = d == 0 ? uc[k] : d == 1 ? vc[k] : wc[k];
+          a[k * pos_step[d]] = (op_t)(Ed * (qc[k] * (double)vel));
+        }
+        op_t *r = As + d * AREA + (R + ij) * KPAD;
+        r[NQ] = fl[face_m[d]];
+        r[NQ + 1] = fl[face_p[d]];
+#pragma unroll
+        for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
+      }
     }
-    const int rd = col * KPAD + kq * KSTEPS;  // A read offset of this lane in a tile
-    const int wr = P::RSTRIDE_Q * kq * KPAD + col;  // result offset of value 0
+    // RK base column, in flight during the GEMMs
+    if (active && use_q0) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) q0v[k] = q0_[base + k * NQ2];
+    }
+    __syncthreads();
+
+    // ---- 3. MFMA GEMMs over (direction, 16-pencil tile) jobs ---------------
+    // Rows of A beyond the block's active pencils hold stale LDS; each output
+    // row depends only on its own A row, and those rows are never read back.
 #pragma unroll
     for (int jj = 0; jj < C::JPW; ++jj) {
       const int job = wave + jj * C::WAVES;
       if (job >= C::JOBS) break;  // wave-uniform
       const int d = job / C::MTILES, t = job - d * C::MTILES;
-      op_t *tile = As + d * AREA + t * 16 * KPAD;
-      typename P::acc_t acc = {0, 0, 0, 0};
+      mfma_tile<P, KSTEPS, KPAD, NQ>(As + d * AREA + t * 16 * KPAD, bop, rd, wr, col);
+    }
+    __syncthreads();
+
+    // ---- 4a. FP64 sum of the three directions ------------------------------
+    if (active) {
 #pragma unroll
-      for (int ks = 0; ks < KSTEPS; ++ks) acc = P::mfma(tile[rd + ks], bop[ks], acc);
-      if (col < NQ) {
+      for (int k = 0; k < NQ; ++k)
+        tend[k] = ((double)As[pos_off[0] + k * pos_step[0]] +
+                   (double)As[AREA + pos_off[1] + k * pos_step[1]]) +
+                  (double)As[2 * AREA + pos_off[2] + k * pos_step[2]];
+    }
+  } else {
+    // ---- 2b-4a, one direction at a time through one LDS area ---------------
 #pragma unroll
-        for (int t4 = 0; t4 < 4; ++t4) tile[wr + t4 * P::RSTRIDE_T * KPAD] = acc[t4];
+    for (int d = 0; d < 3; ++d) {
+      if (active) {
+        const double Ed = g[kGeoEscale + d];
+        const vel_t *vel = velp[d];
+        op_t *a = As + pos_off[d];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+          a[k * pos_step[d]] = (op_t)(Ed * (qc[k] * (double)vel[base + k * NQ2]));
+        op_t *r = As + (R + ij) * KPAD;
+        r[NQ] = fl[face_m[d]];
+        r[NQ + 1] = fl[face_p[d]];
+#pragma unroll
+        for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
       }
+      if (d == 2 && active && use_q0) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) q0v[k] = q0_[base + k * NQ2];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int jj = 0; jj < C::JPW; ++jj) {
+        const int t = wave + jj * C::WAVES;
+        if (t >= C::MTILES) break;  // wave-uniform
+        mfma_tile<P, KSTEPS, KPAD, NQ>(As + t * 16 * KPAD, bop, rd, wr, col);
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+          const double x = (double)As[pos_off[d] + k * pos_step[d]];
+          tend[k] = d == 0 ? x : tend[k] + x;
+        }
+      }
+      if (d < 2) __syncthreads();  // the next direction overwrites the area
     }
   }
-  __syncthreads();
 
-  // ---- 4. FP64 combine and RK epilogue -------------------------------------
+  // ---- 4b. RK epilogue (FP64) -------------------------------------------------
   if (active) {
-    const int R = el * NQ2;
-    const op_t *ax = Ax + (R + j) * KPAD + i;
-    const op_t *ay = Ay + (R + i) * KPAD + j;
-    const op_t *az = Az + (R + ij) * KPAD;
     const double dtv = dt_geo_ ? g[kGeoDt] : dt_;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      const double tend =
-          -(((double)ax[k * NQ * KPAD] + (double)ay[k * NQ * KPAD]) + (double)az[k]);
+      const double td = -tend[k];
       const int64_t idx = base + (int64_t)k * NQ2;
       if (!rk) {
-        out_[idx] = tend;
+        out_[idx] = td;
       } else {
-        const double upd = cb_ * (qc[k] + dtv * tend);
+        const double upd = cb_ * (qc[k] + dtv * td);
         out_[idx] = use_q0 ? ca_ * q0v[k] + upd : upd;
       }
     }
   }
 }
 
-template <class P, int NQ, int EPB>
+template <class P, int NQ, int EPB, bool SEQ>
 int prepare_shape() {
   static int prepared_dev = -1;
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)MP<P, NQ, EPB>::LDS_BYTES));
+                                (int)MP<P, NQ, EPB, SEQ>::LDS_BYTES));
     prepared_dev = dev;
   }
   return DGK_OK;
 }
 
-template <class P, int NQ, int EPB, class Args>
+template <class P, int NQ, int EPB, bool SEQ, class Args>
 int launch_shape(const Args &a, hipStream_t s) {
-  using T = MP<P, NQ, EPB>;
+  using T = MP<P, NQ, EPB, SEQ>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  DGK_TRY((prepare_shape<P, NQ, EPB>()));
-  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB>), dim3((unsigned)nblk), dim3(T::THREADS),
+  DGK_TRY((prepare_shape<P, NQ, EPB, SEQ>()));
+  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ>), dim3((unsigned)nblk), dim3(T::THREADS),
                      T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr, a.vmapP, a.D,
                      a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode,
                      a.axis_aligned, a.dt_geo);
@@ -331,24 +419,43 @@ int launch_shape(const Args &a, hipStream_t s) {
   return DGK_OK;
 }
 
-// Tile-shape candidate selected per Nq (index into mp_epb) for the mixed form.
+// Candidate c of order NQ (compile-time), as a kernel instantiation.
+template <class P, int NQ, int CAND>
+struct Cand {
+  static constexpr int K = kind_of<P, NQ>(CAND);
+  static constexpr int EPB = kind_epb<NQ>(K < 0 ? 0 : K);
+  static constexpr bool SEQ = kind_seq<NQ>(K < 0 ? 0 : K);
+};
+
+// Tile-shape candidate selected per Nq for the mixed form.
 int g_mp_shape[kMaxNq + 1] = {0};
 
 template <class P, int NQ>
 int prepare_nq_all() {
-  DGK_TRY((prepare_shape<P, NQ, mp_epb<NQ>(0)>()));
-  if constexpr (mp_count<P, NQ>() > 1) DGK_TRY((prepare_shape<P, NQ, mp_epb<NQ>(1)>()));
-  if constexpr (mp_count<P, NQ>() > 2) DGK_TRY((prepare_shape<P, NQ, mp_epb<NQ>(2)>()));
+  DGK_TRY((prepare_shape<P, NQ, Cand<P, NQ, 0>::EPB, Cand<P, NQ, 0>::SEQ>()));
+  if constexpr (ncand<P, NQ>() > 1)
+    DGK_TRY((prepare_shape<P, NQ, Cand<P, NQ, 1>::EPB, Cand<P, NQ, 1>::SEQ>()));
+  if constexpr (ncand<P, NQ>() > 2)
+    DGK_TRY((prepare_shape<P, NQ, Cand<P, NQ, 2>::EPB, Cand<P, NQ, 2>::SEQ>()));
   return DGK_OK;
 }
 
 template <class P, int NQ, class Args>
 int launch_nq_cand(int cand, const Args &a, hipStream_t s) {
-  if constexpr (mp_count<P, NQ>() > 1)
-    if (cand == 1) return launch_shape<P, NQ, mp_epb<NQ>(1)>(a, s);
-  if constexpr (mp_count<P, NQ>() > 2)
-    if (cand == 2) return launch_shape<P, NQ, mp_epb<NQ>(2)>(a, s);
-  return launch_shape<P, NQ, mp_epb<NQ>(0)>(a, s);
+  if constexpr (ncand<P, NQ>() > 1)
+    if (cand == 1) return launch_shape<P, NQ, Cand<P, NQ, 1>::EPB, Cand<P, NQ, 1>::SEQ>(a, s);
+  if constexpr (ncand<P, NQ>() > 2)
+    if (cand == 2) return launch_shape<P, NQ, Cand<P, NQ, 2>::EPB, Cand<P, NQ, 2>::SEQ>(a, s);
+  return launch_shape<P, NQ, Cand<P, NQ, 0>::EPB, Cand<P, NQ, 0>::SEQ>(a, s);
+}
+
+template <class P, int NQ>
+int shapes_of(int *epb) {
+  constexpr int n = ncand<P, NQ>();
+  if constexpr (n > 0) epb[0] = Cand<P, NQ, 0>::EPB;
+  if constexpr (n > 1) epb[1] = Cand<P, NQ, 1>::EPB;
+  if constexpr (n > 2) epb[2] = Cand<P, NQ, 2>::EPB;
+  return n;
 }
 
 }  // namespace
@@ -388,15 +495,6 @@ int launch_tile_f64(int Nq, int cand, const StageArgs &a, hipStream_t s) {
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
 }
-
-namespace {
-template <class P, int NQ>
-int shapes_of(int *epb) {
-  const int n = mp_count<P, NQ>();
-  for (int c = 0; c < n; ++c) epb[c] = mp_epb<NQ>(c);
-  return n;
-}
-}  // namespace
 
 int stage_mp_shapes(int Nq, int *epb, int n) {
   if (Nq < kMinNq || Nq > kMaxNq || !epb || n < 3) return 0;

@@ -144,16 +144,17 @@ def test_serve_request_stream_ranges():
 def test_tile_shapes_host_side():
     assert dgk.tile_shapes("HIP_MP", 7) == [1, 2, 4]
     assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16]
-    assert dgk.tile_shapes("HIP_MP", 15) == [1, 2]
+    # p >= 8: one element, two elements, one element direction by direction
+    assert dgk.tile_shapes("HIP_MP", 15) == [1, 2, 1]
     # FP64: the VALU kernel, then the FP64 MFMA kernel's shapes (two elements
     # of p = 15 would need more than 160 KiB of LDS)
     assert dgk.tile_shapes("HIP", 7) == [1, 1, 2, 4]
-    assert dgk.tile_shapes("HIP", 15) == [1, 1]
+    assert dgk.tile_shapes("HIP", 15) == [1, 1, 1]
     with pytest.raises(dgk.DgkError) as ei:
-        dgk.tile_select("HIP", 15, 2)
+        dgk.tile_select("HIP", 15, 3)
     assert ei.value.code == dgk.ERR_ARG
     with pytest.raises(dgk.DgkError) as ei:
-        dgk.tile_select("HIP_MP", 15, 2)
+        dgk.tile_select("HIP_MP", 15, 3)
     assert ei.value.code == dgk.ERR_ARG
     with pytest.raises(dgk.DgkError) as ei:
         dgk.tile_shapes("GENERAL", 7)
@@ -162,9 +163,10 @@ def test_tile_shapes_host_side():
     assert dgk.tile_selected("HIP_MP", 7) == 2
     dgk.tile_select("HIP_MP", 7, -1)
     assert dgk.tile_selected("HIP_MP", 7) == 0
-    # FP64 defaults: the VALU kernel at p <= 7 and 10..12, the MFMA kernel else
-    assert [dgk.tile_selected("HIP", p) for p in range(1, 16)] == [0] * 7 + [1, 1, 0, 0, 0, 1, 1, 1]
+    # FP64 defaults: the VALU kernel at p <= 7 and 10, 11, the
+    # direction-by-direction MFMA kernel (the last candidate) else
+    assert [dgk.tile_selected("HIP", p) for p in range(1, 16)] == [0] * 7 + [3, 3, 0, 0, 3, 3, 2, 2]
     dgk.tile_select("HIP", 13, 0)
     assert dgk.tile_selected("HIP", 13) == 0
     dgk.tile_select("HIP", 13, -1)
-    assert dgk.tile_selected("HIP", 13) == 1
+    assert dgk.tile_selected("HIP", 13) == 3
