@@ -18,7 +18,10 @@ The JSON line also carries
                 ensemble (512 seeded instances, Ne/axis in {4,6,8,12,16},
                 p in {3,5,7,11}, 50 steps, graph-replayed) in FP64 and mixed
                 precision against the plain FP64 path run instance by
-                instance; serving: the first --serve requests of the seeded
+                instance; C3 (16^3, p=15, 100 steps) in both forms with the
+                mixed form's error against FP64; per-order stage rates of
+                both forms (p = 1, 3, ..., 15 at about 1.7e7 DOF) against
+                the HBM and MFMA peaks; serving: the first --serve requests of the seeded
                 Poisson stream (Ne/axis 4..32, p 2..15, nstep 10..500, half
                 of them in the 1e-5 class) at --serve-rate requests/s, with
                 p50/p99 latency, DOF-updates/s and a sample of results checked
@@ -81,15 +84,17 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
 
 
-def traffic_from_profile(form):
-    """HBM bytes per launch measured by rocprofv3 PMC runs (profiles/)."""
-    name = "stage_f64" if form == "HIP" else "stage_mp"
-    f = ROOT / "profiles" / f"traffic_{name}_c2_32cube_p7.json"
-    if f.exists():
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` on this workload, measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (None when
+    no summary covers this kernel)."""
+    for f in sorted((ROOT / "profiles").glob("traffic_*_c2_32cube_p7.json")):
         try:
-            return json.loads(f.read_text()).get("bytes_per_launch")
-        except Exception:
-            return None
+            d = json.loads(f.read_text())
+        except (OSError, ValueError):
+            continue
+        if d.get("kernel") == kernel:
+            return d.get("bytes_per_launch")
     return None
 
 
@@ -98,6 +103,8 @@ def time_solver(dgk, inputs, form, steps, warmup):
     dof = wl.Ne * wl.Np
     s = dgk.Solver.structured(form, wl.NeX, wl.NeY, wl.NeZ, wl.p)
     s.set_fields(wl.q, wl.u, wl.v, wl.w)
+    tune = s.autotune(reps=10)
+    tune["kernel"] = dgk.kernel_name(form, wl.p, tune["best"])
     s.step(wl.dt, warmup)
     s.synchronize()
     s.timer_reset()
@@ -114,7 +121,7 @@ def time_solver(dgk, inputs, form, steps, warmup):
     secs = t1 - t0
     stage_s, stage_n = tm["Stage (fused)"]
     return wl, dict(value=dof * steps / secs, seconds=secs, launch_s=stage_s / max(stage_n, 1),
-                    launches=stage_n, timers=tm, minmax=[qmin, qmax], dof=dof)
+                    launches=stage_n, timers=tm, minmax=[qmin, qmax], dof=dof, tune=tune)
 
 
 def batch_leg(dgk, inputs):
@@ -177,6 +184,76 @@ def batch_leg(dgk, inputs):
     return out
 
 
+def flops_per_dof_update(p):
+    """Useful FLOPs of one DOF advanced one SSP-RK3 step: per stage 6 Nq for
+    the three contractions, 120/Nq for the six face lifts and fluxes spread
+    over the element, 25 for the volume fluxes, the sum and the RK update."""
+    nq = p + 1
+    return 3.0 * (6 * nq + 25 + 120.0 / nq)
+
+
+def per_order_leg(dgk, orders=(1, 3, 5, 7, 9, 11, 13, 15), steps=20):
+    """Stage-kernel rate of both forms at every other order on a periodic box
+    of about 1.68e7 DOF (Gaussian blob, constant velocity), default tile
+    shapes: achieved algorithmic HBM rate and useful FLOP rate against the
+    MI355X peaks (HBM 8 TB/s; MFMA FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s)."""
+    out = []
+    for p in orders:
+        np_ = (p + 1) ** 3
+        ne = max(2, round((16_777_216 / np_) ** (1.0 / 3.0)))
+        dof = ne ** 3 * np_
+        for form in ("HIP", "HIP_MP"):
+            with dgk.Solver.structured(form, ne, ne, ne, p) as s:
+                s.init_gaussian((0.3, 0.6, 0.2), 0.1, (0.9, -0.4, 0.3))
+                dt = 0.05 * (2.0 / ne) / np_ ** (1.0 / 3.0)
+                s.step(dt, 2)
+                s.synchronize()
+                s.timer_reset()
+                s.set_kernel_timing(True)
+                s.step(dt, steps)
+                s.synchronize()
+                sec, n = s.timers()["Stage (fused)"]
+            launch = sec / n
+            gbs = BYTES_PER_DOF_UPDATE[form] / 3.0 * dof / launch / 1e9
+            tfs = flops_per_dof_update(p) / 3.0 * dof / launch / 1e12
+            peak = 78.6 if form == "HIP" else 157.3
+            shape = dgk.tile_shapes(form, p)[dgk.tile_selected(form, p)]
+            out.append({"p": p, "form": form, "ne": ne, "dof": dof, "avg_launch_us": 1e6 * launch,
+                        "tile_candidate": dgk.tile_selected(form, p), "elements_per_workgroup": shape,
+                        "DOF-updates/s": dof / (3.0 * launch), "alg_GBs": gbs,
+                        "frac_hbm": gbs / HBM_PEAK_GBS, "useful_TFLOPs": tfs,
+                        "frac_mfma_peak": tfs / peak})
+    return out
+
+
+def c3_leg(dgk, inputs):
+    """16^3 elements at p=15 (sin(6 pi x) sin(6 pi y) sin(6 pi z), velocity
+    (1,1,1), 100 steps) in both forms: throughput, and the mixed form's error
+    against the FP64 device path on the same inputs (the FP64 path is held to
+    1e-12 of the reference by the parity tests)."""
+    wl = inputs.c3_sine()
+    dof = wl.Ne * wl.Np
+    out, fields = {"workload": f"16^3 periodic hexahedra, p=15, {dof} DOF, {wl.nstep} steps"}, {}
+    for form in ("HIP", "HIP_MP"):
+        with dgk.Solver.structured(form, wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+            s.set_fields(wl.q, wl.u, wl.v, wl.w)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(wl.dt, wl.nstep)
+            s.synchronize()
+            t1 = time.perf_counter()
+            fields[form] = s.get_field(dgk.FIELD_Q)
+        out[form] = {"DOF-updates/s": dof * wl.nstep / (t1 - t0),
+                     "ms_per_step": 1e3 * (t1 - t0) / wl.nstep,
+                     "elements_per_workgroup": dgk.tile_shapes(form, wl.p)[dgk.tile_selected(form, wl.p)],
+                     "tile_candidate": dgk.tile_selected(form, wl.p)}
+    ref, mp = fields["HIP"], fields["HIP_MP"]
+    out["mp_rel_l2_vs_fp64"] = float(np.linalg.norm(mp - ref) / np.linalg.norm(ref))
+    out["mp_max_rel_vs_fp64"] = float(np.abs(mp - ref).max() / np.abs(ref).max())
+    out["speedup_mp_vs_fp64"] = out["HIP_MP"]["DOF-updates/s"] / out["HIP"]["DOF-updates/s"]
+    return out
+
+
 def cpu_baseline(wl, steps):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -214,10 +291,7 @@ def main():
     dof = r["dof"]
     bytes_per_launch = BYTES_PER_DOF_UPDATE[args.form] / 3.0 * dof
     achieved = bytes_per_launch / r["launch_s"] / 1e9
-    if args.form == "HIP":
-        kname = "stage_f64<8>"
-    else:
-        kname = f"stage_mp<8, {dgk.tile_shapes('HIP_MP', 7)[dgk.tile_selected('HIP_MP', 7)]}>"
+    kname = r["tune"]["kernel"]
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -236,9 +310,10 @@ def main():
                    "NeX": wl.NeX, "p": wl.p, "dof": dof, "dt": wl.dt, "form": args.form},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(args.form), "kernel": kname,
+                     "traffic": traffic_from_profile(kname), "kernel": kname,
                      "avg_launch_us": 1e6 * r["launch_s"], "launches": r["launches"],
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "autotune_us_per_launch": r["tune"]["us_per_launch"]},
         "cpu_baseline": None,
         "extra": {"q_minmax_after": r["minmax"], "device_timers": r["timers"]},
     }
@@ -250,6 +325,8 @@ def main():
             "achieved_GBs": BYTES_PER_DOF_UPDATE[other] / 3.0 * dof / r2["launch_s"] / 1e9,
             "ratio_vs_" + args.form: r2["value"] / r["value"]}
         line["extra"]["batched_ensemble"] = batch_leg(dgk, inputs)
+        line["extra"]["c3_p15"] = c3_leg(dgk, inputs)
+        line["extra"]["per_order"] = per_order_leg(dgk)
     if args.serve:
         from dgk import serving
 

@@ -219,16 +219,17 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  *           The default is the VALU kernel for p <= 7 and p = 10, 11, and the
  *           direction-by-direction MFMA shape (the last candidate) for p = 8,
  *           9 and 12..15.
- * dgk_tile_select(form, p, -1) restores the default.  The selection holds
- * for every later launch of that form and order in the process, except that
- * launches with a given flux (dgk_cal_dqdt) or a stored, non-separable Lift
- * always use the VALU kernels.  dgk_autotune times each candidate
- * with HIP events over reps RK-stage launches on the solver's own mesh
- * (writing only the solver's scratch rotation buffer) and selects the
- * fastest; us_per_launch[c] receives candidate c's time.  dgk_batch_autotune
- * does this for every order group of a batch and reports per group: the
- * order, the chosen candidate, its time and the default's time (the batch's
- * captured step is re-captured at the next dgk_batch_step).  No counterpart
+ * dgk_tile_select sets the process-wide choice for that form and order
+ * (-1 restores the default); it holds for every solver that has not been
+ * autotuned.  Launches with a given flux (dgk_cal_dqdt) or a stored,
+ * non-separable Lift always use the VALU kernels.  dgk_autotune times each
+ * candidate with HIP events over reps RK-stage launches on the solver's own
+ * mesh (writing only the solver's scratch rotation buffer) and keeps the
+ * fastest for that solver alone, since the best shape depends on the mesh;
+ * us_per_launch[c] receives candidate c's time.  dgk_batch_autotune does this
+ * for every order group of a batch and reports per group: the order, the
+ * chosen candidate, its time and the default's time (the batch's captured
+ * step is re-captured at the next dgk_batch_step).  No counterpart
  * in the reference, whose OPT1 kernels are generated per order
  * (mod_dg_optr_kernel_opt1.F90.erb).  The MFMA shapes of one form give the
  * same bits as each other; the FP64 VALU and MFMA kernels differ by
@@ -238,6 +239,8 @@ int dgk_tile_select(int evaltype, int p, int cand);
 int dgk_tile_selected(int evaltype, int p, int *cand);
 int dgk_autotune(dgk_solver *s, int reps, int *best, double *us_per_launch, int max,
                  int *n);
+/* Candidate the solver's launches use (its own, else the process-wide one). */
+int dgk_solver_tile(const dgk_solver *s, int *cand);
 int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_best,
                        double *us_default, int max, int *n);
 

@@ -123,6 +123,7 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
   a.ell = s->ell; a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.Ne = s->Ne;
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
   a.dt_geo = 0;
+  a.tile = s->tile;
   return a;
 }
 
@@ -139,6 +140,7 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
     a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.Ne = s->Ne;
     a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
     a.dt_geo = dt_geo;
+    a.tile = s->tile;
     return launch_stage_mp(s->Nq, a, stream);
   }
   StageArgs a = stage_args(s, qin, q0, out, mode, ca, cb, dt);
@@ -966,16 +968,20 @@ int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
   int arg = 0;
   double t_best = 0.0;
   for (int c = 0; c < nc; ++c) {
-    DGK_TRY(tile_select(s->evaltype, s->Nq, c));
+    s->tile = c;
     double t = 0.0;
-    DGK_TRY(time_stage(s, reps, &t));
+    const int rc = time_stage(s, reps, &t);
+    if (rc != DGK_OK) {
+      s->tile = -1;
+      return rc;
+    }
     if (us && c < max) us[c] = t;
     if (c == 0 || t < t_best) {
       t_best = t;
       arg = c;
     }
   }
-  DGK_TRY(tile_select(s->evaltype, s->Nq, arg));
+  s->tile = arg;
   if (best) *best = arg;
   if (n) *n = nc;
   return DGK_OK;
@@ -1007,6 +1013,14 @@ int dgk_tile_selected(int evaltype, int p, int *cand) {
   DGK_TRY(check_form_order(evaltype, p));
   if (!cand) return set_error(DGK_ERR_ARG, "NULL argument");
   *cand = tile_selected(evaltype, p + 1);
+  return DGK_OK;
+}
+
+int dgk_solver_tile(const dgk_solver *s, int *cand) {
+  clear_error();
+  DGK_TRY(check_solver(s));
+  if (!cand) return set_error(DGK_ERR_ARG, "NULL argument");
+  *cand = s->tile >= 0 ? s->tile : tile_selected(s->evaltype, s->Nq);
   return DGK_OK;
 }
 

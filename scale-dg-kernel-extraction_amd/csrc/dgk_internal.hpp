@@ -162,6 +162,7 @@ struct StageArgs {
   int mode;               // 0 = tendency, 1 = RK stage
   int axis_aligned;       // face normals along the face axes (see MeshTables)
   int dt_geo;             // RK time step from the geometry record (per instance)
+  int tile;               // tile-shape candidate, -1: the process-wide selection
 };
 
 // Mixed-precision operands (FP32 copies of the constant inputs).
@@ -182,6 +183,7 @@ struct StageArgsMP {
   int mode;
   int axis_aligned;
   int dt_geo;
+  int tile;               // tile-shape candidate, -1: the process-wide selection
 };
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);

@@ -580,8 +580,8 @@ int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
-  if (Nq >= kMinNq && Nq <= kMaxNq && g_f64_shape[Nq] > 0 && !a.flux_in && a.ell)
-    return launch_tile_f64(Nq, g_f64_shape[Nq] - 1, a, s);
+  const int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0 : a.tile >= 0 ? a.tile : g_f64_shape[Nq];
+  if (cand > 0 && !a.flux_in && a.ell) return launch_tile_f64(Nq, cand - 1, a, s);
   switch (Nq) {
   case 2: return launch_nq<2>(a, s);
   case 3: return launch_nq<3>(a, s);

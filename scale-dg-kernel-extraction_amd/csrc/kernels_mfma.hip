@@ -478,7 +478,7 @@ int prepare_stage_mp(int Nq) {
 
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
-#define DGK_CALL(N) launch_nq_cand<PrecMixed, N>(g_mp_shape[N], a, s)
+#define DGK_CALL(N) launch_nq_cand<PrecMixed, N>(a.tile >= 0 ? a.tile : g_mp_shape[N], a, s)
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
 }

@@ -37,6 +37,7 @@ struct dgk_solver {
   double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
   int32_t *order = nullptr;  // element processing order
   int axis_aligned = 0;
+  int tile = -1;             // tile-shape candidate chosen for this mesh (-1: process default)
   double *x1D = nullptr;     // LGL nodes (device), for device-side initial conditions
   BoxShape box;              // structured box (NeX == 0 when built from arrays)
   int32_t *nbr = nullptr, *vmapP = nullptr, *halo = nullptr;

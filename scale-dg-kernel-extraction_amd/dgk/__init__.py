@@ -77,6 +77,7 @@ _SIGNATURES = {
     "dgk_tile_select": (_I, [_I, _I, _I]),
     "dgk_tile_selected": (_I, [_I, _I, _P]),
     "dgk_autotune": (_I, [_P, _I, _P, _P, _I, _P]),
+    "dgk_solver_tile": (_I, [_P, _P]),
     "dgk_batch_autotune": (_I, [_P, _I, _P, _P, _P, _P, _I, _P]),
     "dgk_serve_create": (_I, [_S, _L, _I, _I, _P]),
     "dgk_serve_footprint": (_I, [_P, _I, _I, _P]),
@@ -150,6 +151,21 @@ def tile_shapes(evaltype, p: int) -> list:
     e, n = np.zeros(4, np.int32), C.c_int(0)
     _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 4, C.byref(n)))
     return e[:n.value].tolist()
+
+
+def kernel_name(evaltype, p: int, cand: int) -> str:
+    """Name of the stage kernel that tile candidate `cand` launches."""
+    if isinstance(evaltype, str):
+        evaltype = evaltype_id(evaltype)
+    nq = p + 1
+    if evaltype == EVAL_HIP and cand == 0:
+        return f"stage_f64<{nq}>"
+    shapes = tile_shapes(evaltype, p)
+    mfma = shapes[1:] if evaltype == EVAL_HIP else shapes
+    mc = cand - 1 if evaltype == EVAL_HIP else cand
+    seq = nq * nq > 64 and mc == len(mfma) - 1 and len(mfma) > 1 and mfma[mc] == 1
+    prec = "PrecF64" if evaltype == EVAL_HIP else "PrecMixed"
+    return f"stage_mfma<{prec}, {nq}, {mfma[mc]}, {'true' if seq else 'false'}>"
 
 
 def tile_select(evaltype, p: int, cand: int) -> None:
@@ -342,9 +358,15 @@ class Solver:
         _check(lib().dgk_init_gaussian(self._h, *(float(c) for c in centre), float(sigma),
                                        *(float(c) for c in vel)))
 
+    def tile(self) -> int:
+        """Tile-shape candidate this solver's launches use."""
+        c = C.c_int(0)
+        _check(lib().dgk_solver_tile(self._h, C.byref(c)))
+        return c.value
+
     def autotune(self, reps: int = 10) -> dict:
         """Time every tile shape of this solver's form and order on its mesh and
-        select the fastest (process-wide for that form and order)."""
+        keep the fastest for this solver."""
         best, n = C.c_int(0), C.c_int(0)
         us = np.zeros(4)
         _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 4, C.byref(n)))

@@ -130,8 +130,14 @@ def test_autotune_solver_and_batch():
         assert 0 <= r["best"] < len(dgk.tile_shapes("HIP_MP", wl.p))
         assert len(r["us_per_launch"]) == len(dgk.tile_shapes("HIP_MP", wl.p))
         assert all(t > 0 for t in r["us_per_launch"])
-        assert dgk.tile_selected("HIP_MP", wl.p) == r["best"]
+        assert dgk.tile_selected("HIP_MP", wl.p) == 0  # the choice is the solver's own
         assert np.array_equal(s.get_field(dgk.FIELD_Q), before)  # state untouched
+        s.step(wl.dt, 2)
+        tuned = s.get_field(dgk.FIELD_Q)
+    with dgk.Solver.structured("HIP_MP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        s.step(wl.dt, 4)
+        assert np.array_equal(s.get_field(dgk.FIELD_Q), tuned)  # shapes give the same bits
     ens = inputs.ensemble(n=6, seed=31, ne_choices=(2, 3), p_choices=(3, 7), nstep=4)
     with dgk.Batch("HIP_MP", ens) as b, dgk.Batch("HIP_MP", ens) as ref:
         for bb in (b, ref):
@@ -146,5 +152,3 @@ def test_autotune_solver_and_batch():
         b.step(1)  # re-captured with the tuned shapes
         for k in range(len(ens)):
             assert np.array_equal(b.get_field(k), ref.get_field(k)), k
-    for p in (3, 7, wl.p):
-        dgk.tile_select("HIP_MP", p, -1)
