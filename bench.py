@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=120,
                     help="time steps of the bounded CPU baseline sample")
     ap.add_argument("--no-extra", action="store_true", help="headline only")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the headline solver on the default tile shape (profiling runs, "
+                         "whose serialised launches would distort the autotuner's timings)")
     ap.add_argument("--serve", type=int, default=150,
                     help="requests of the seeded Poisson stream to serve (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=6.0,
@@ -98,12 +101,12 @@ def traffic_from_profile(kernel):
     return None
 
 
-def time_solver(dgk, inputs, form, steps, warmup):
+def time_solver(dgk, inputs, form, steps, warmup, autotune=True):
     wl = inputs.c2_gaussian()
     dof = wl.Ne * wl.Np
     s = dgk.Solver.structured(form, wl.NeX, wl.NeY, wl.NeZ, wl.p)
     s.set_fields(wl.q, wl.u, wl.v, wl.w)
-    tune = s.autotune(reps=10)
+    tune = s.autotune(reps=10) if autotune else {"best": s.tile(), "us_per_launch": None}
     tune["kernel"] = dgk.kernel_name(form, wl.p, tune["best"])
     s.step(wl.dt, warmup)
     s.synchronize()
@@ -287,7 +290,8 @@ def main():
     import dgk
     from dgk import inputs
 
-    wl, r = time_solver(dgk, inputs, args.form, args.steps, args.warmup)
+    wl, r = time_solver(dgk, inputs, args.form, args.steps, args.warmup,
+                        autotune=not args.no_autotune)
     dof = r["dof"]
     bytes_per_launch = BYTES_PER_DOF_UPDATE[args.form] / 3.0 * dof
     achieved = bytes_per_launch / r["launch_s"] / 1e9

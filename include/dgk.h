@@ -225,7 +225,8 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  * non-separable Lift always use the VALU kernels.  dgk_autotune times each
  * candidate with HIP events over reps RK-stage launches on the solver's own
  * mesh (writing only the solver's scratch rotation buffer) and keeps the
- * fastest for that solver alone, since the best shape depends on the mesh;
+ * fastest for that solver alone, since the best shape depends on the mesh
+ * (the default stays unless another shape is at least 2% faster);
  * us_per_launch[c] receives candidate c's time.  dgk_batch_autotune does this
  * for every order group of a batch and reports per group: the order, the
  * chosen candidate, its time and the default's time (the batch's captured

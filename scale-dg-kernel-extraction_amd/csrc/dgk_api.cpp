@@ -961,26 +961,27 @@ int time_stage(dgk_solver *s, int reps, double *us) {
   return rc;
 }
 
+constexpr double kTuneMargin = 0.02;
+
 int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
   int epb[kMaxTileShapes];
   const int nc = tile_shapes(s->evaltype, s->Nq, epb);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "no tile shapes for this form and order");
+  double t[kMaxTileShapes] = {0, 0, 0, 0};
   int arg = 0;
-  double t_best = 0.0;
   for (int c = 0; c < nc; ++c) {
     s->tile = c;
-    double t = 0.0;
-    const int rc = time_stage(s, reps, &t);
+    const int rc = time_stage(s, reps, &t[c]);
     if (rc != DGK_OK) {
       s->tile = -1;
       return rc;
     }
-    if (us && c < max) us[c] = t;
-    if (c == 0 || t < t_best) {
-      t_best = t;
-      arg = c;
-    }
+    if (us && c < max) us[c] = t[c];
+    if (t[c] < t[arg]) arg = c;
   }
+  // keep the default unless another shape is at least kTuneMargin faster
+  const int def = tile_selected(s->evaltype, s->Nq);
+  if (def >= 0 && def < nc && t[arg] > (1.0 - kTuneMargin) * t[def]) arg = def;
   s->tile = arg;
   if (best) *best = arg;
   if (n) *n = nc;
