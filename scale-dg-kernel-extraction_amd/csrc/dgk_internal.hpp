@@ -195,7 +195,7 @@ int stage_mp_shapes(int Nq, int *epb, int n);
 int stage_mp_select(int Nq, int cand);
 int stage_mp_selected(int Nq);
 // FP64: candidate 0 is the VALU kernel, 1.. the FP64 MFMA kernel's shapes.
-constexpr int kMaxTileShapes = 4;
+constexpr int kMaxTileShapes = 5;
 int stage_f64_shapes(int Nq, int *epb, int n);
 int stage_f64_select(int Nq, int cand);
 int stage_f64_selected(int Nq);

@@ -181,7 +181,9 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   }
   if (active && use_q0) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) q0v[k] = q0_[base + (int64_t)k * NQ2];
+    // the RK base and the output are touched once per launch: non-temporal,
+    // so L2 keeps the neighbour traces
+    for (int k = 0; k < NQ; ++k) q0v[k] = __builtin_nontemporal_load(q0_ + base + (int64_t)k * NQ2);
   }
   __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
   __syncthreads();
@@ -259,10 +261,10 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
       const double tend = -(((Ex * dx + Ey * dy) + Ez * dz) + lift);
       const int64_t idx = base + (int64_t)k * NQ2;
       if (!rk) {
-        out_[idx] = tend;
+        __builtin_nontemporal_store(tend, out_ + idx);
       } else {
         const double upd = cb_ * (qc[k] + (dt_geo_ ? g[kGeoDt] : dt_) * tend);
-        out_[idx] = use_q0 ? ca_ * q0v[k] + upd : upd;
+        __builtin_nontemporal_store(use_q0 ? ca_ * q0v[k] + upd : upd, out_ + idx);
       }
     }
   }

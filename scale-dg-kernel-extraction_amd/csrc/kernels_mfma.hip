@@ -104,16 +104,17 @@ struct MP {
   static constexpr int JPW = (JOBS + WAVES - 1) / WAVES;
 };
 
-// Candidate kinds per order: one, two or four wavefronts of whole elements
-// for Nq^2 <= 64; above, one element, two elements, and one element
-// direction by direction.  Kinds whose LDS exceeds a CU's 160 KiB are
-// dropped; the remaining ones are the candidates 0, 1, ... (0 is the
-// default).
+// Candidate kinds per order: for Nq^2 <= 64 one, two or four wavefronts of
+// whole elements, then one wavefront direction by direction; above, one
+// element, two elements, and one element direction by direction.  Kinds
+// whose LDS exceeds a CU's 160 KiB are dropped; the remaining ones are the
+// candidates 0, 1, ... (0 is the default).
+constexpr int kKinds = 4;
 template <int NQ>
 constexpr int kind_epb(int k) {
   constexpr int NQ2 = NQ * NQ;
   if (NQ2 <= 64) {
-    const int waves = k == 0 ? 1 : k == 1 ? 2 : 4;
+    const int waves = k == 1 ? 2 : k == 2 ? 4 : 1;
     const int e = 64 * waves / NQ2;
     return e < 1 ? 1 : e;
   }
@@ -121,23 +122,26 @@ constexpr int kind_epb(int k) {
 }
 template <int NQ>
 constexpr bool kind_seq(int k) {
-  return NQ * NQ > 64 && k == 2;
+  return NQ * NQ > 64 ? k == 2 : k == 3;
 }
 template <class P, int NQ>
 constexpr bool kind_ok(int k) {
+  if (NQ * NQ > 64 && k == 3) return false;
   return k == 0   ? MP<P, NQ, kind_epb<NQ>(0), kind_seq<NQ>(0)>::LDS_BYTES <= 160 * 1024
          : k == 1 ? MP<P, NQ, kind_epb<NQ>(1), kind_seq<NQ>(1)>::LDS_BYTES <= 160 * 1024
-                  : MP<P, NQ, kind_epb<NQ>(2), kind_seq<NQ>(2)>::LDS_BYTES <= 160 * 1024;
+         : k == 2 ? MP<P, NQ, kind_epb<NQ>(2), kind_seq<NQ>(2)>::LDS_BYTES <= 160 * 1024
+                  : MP<P, NQ, kind_epb<NQ>(3), kind_seq<NQ>(3)>::LDS_BYTES <= 160 * 1024;
 }
 template <class P, int NQ>
 constexpr int ncand() {
-  return (int)kind_ok<P, NQ>(0) + (int)kind_ok<P, NQ>(1) + (int)kind_ok<P, NQ>(2);
+  return (int)kind_ok<P, NQ>(0) + (int)kind_ok<P, NQ>(1) + (int)kind_ok<P, NQ>(2) +
+         (int)kind_ok<P, NQ>(3);
 }
 // kind of candidate c (-1 if none)
 template <class P, int NQ>
 constexpr int kind_of(int c) {
   int n = 0;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < kKinds; ++k) {
     if (!kind_ok<P, NQ>(k)) continue;
     if (n == c) return k;
     ++n;
@@ -307,10 +311,12 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
         for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
       }
     }
-    // RK base column, in flight during the GEMMs
+    // RK base column, in flight during the GEMMs (the RK base and the output
+    // are touched once per launch: non-temporal, so L2 keeps the neighbour
+    // traces)
     if (active && use_q0) {
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) q0v[k] = q0_[base + k * NQ2];
+      for (int k = 0; k < NQ; ++k) q0v[k] = __builtin_nontemporal_load(q0_ + base + k * NQ2);
     }
     __syncthreads();
 
@@ -353,7 +359,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       }
       if (d == 2 && active && use_q0) {
 #pragma unroll
-        for (int k = 0; k < NQ; ++k) q0v[k] = q0_[base + k * NQ2];
+        for (int k = 0; k < NQ; ++k) q0v[k] = __builtin_nontemporal_load(q0_ + base + k * NQ2);
       }
       __syncthreads();
 #pragma unroll
@@ -382,10 +388,10 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       const double td = -tend[k];
       const int64_t idx = base + (int64_t)k * NQ2;
       if (!rk) {
-        out_[idx] = td;
+        __builtin_nontemporal_store(td, out_ + idx);
       } else {
         const double upd = cb_ * (qc[k] + dtv * td);
-        out_[idx] = use_q0 ? ca_ * q0v[k] + upd : upd;
+        __builtin_nontemporal_store(use_q0 ? ca_ * q0v[k] + upd : upd, out_ + idx);
       }
     }
   }
@@ -437,6 +443,8 @@ int prepare_nq_all() {
     DGK_TRY((prepare_shape<P, NQ, Cand<P, NQ, 1>::EPB, Cand<P, NQ, 1>::SEQ>()));
   if constexpr (ncand<P, NQ>() > 2)
     DGK_TRY((prepare_shape<P, NQ, Cand<P, NQ, 2>::EPB, Cand<P, NQ, 2>::SEQ>()));
+  if constexpr (ncand<P, NQ>() > 3)
+    DGK_TRY((prepare_shape<P, NQ, Cand<P, NQ, 3>::EPB, Cand<P, NQ, 3>::SEQ>()));
   return DGK_OK;
 }
 
@@ -446,6 +454,8 @@ int launch_nq_cand(int cand, const Args &a, hipStream_t s) {
     if (cand == 1) return launch_shape<P, NQ, Cand<P, NQ, 1>::EPB, Cand<P, NQ, 1>::SEQ>(a, s);
   if constexpr (ncand<P, NQ>() > 2)
     if (cand == 2) return launch_shape<P, NQ, Cand<P, NQ, 2>::EPB, Cand<P, NQ, 2>::SEQ>(a, s);
+  if constexpr (ncand<P, NQ>() > 3)
+    if (cand == 3) return launch_shape<P, NQ, Cand<P, NQ, 3>::EPB, Cand<P, NQ, 3>::SEQ>(a, s);
   return launch_shape<P, NQ, Cand<P, NQ, 0>::EPB, Cand<P, NQ, 0>::SEQ>(a, s);
 }
 
@@ -455,6 +465,7 @@ int shapes_of(int *epb) {
   if constexpr (n > 0) epb[0] = Cand<P, NQ, 0>::EPB;
   if constexpr (n > 1) epb[1] = Cand<P, NQ, 1>::EPB;
   if constexpr (n > 2) epb[2] = Cand<P, NQ, 2>::EPB;
+  if constexpr (n > 3) epb[3] = Cand<P, NQ, 3>::EPB;
   return n;
 }
 
@@ -497,22 +508,22 @@ int launch_tile_f64(int Nq, int cand, const StageArgs &a, hipStream_t s) {
 }
 
 int stage_mp_shapes(int Nq, int *epb, int n) {
-  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < 3) return 0;
+  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < kKinds) return 0;
 #define DGK_CALL(N) shapes_of<PrecMixed, N>(epb)
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
 }
 
 int tile_f64_shapes(int Nq, int *epb, int n) {
-  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < 3) return 0;
+  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < kKinds) return 0;
 #define DGK_CALL(N) shapes_of<PrecF64, N>(epb)
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
 }
 
 int stage_mp_select(int Nq, int cand) {
-  int e[3];
-  const int nc = stage_mp_shapes(Nq, e, 3);
+  int e[kKinds];
+  const int nc = stage_mp_shapes(Nq, e, kKinds);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
   if (cand == -1) cand = 0;  // the default
   if (cand < 0 || cand >= nc)

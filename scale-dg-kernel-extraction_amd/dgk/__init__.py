@@ -148,8 +148,8 @@ def tile_shapes(evaltype, p: int) -> list:
     """Elements per workgroup of each tile-shape candidate of a stage kernel."""
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
-    e, n = np.zeros(4, np.int32), C.c_int(0)
-    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 4, C.byref(n)))
+    e, n = np.zeros(5, np.int32), C.c_int(0)
+    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 5, C.byref(n)))
     return e[:n.value].tolist()
 
 
@@ -163,7 +163,7 @@ def kernel_name(evaltype, p: int, cand: int) -> str:
     shapes = tile_shapes(evaltype, p)
     mfma = shapes[1:] if evaltype == EVAL_HIP else shapes
     mc = cand - 1 if evaltype == EVAL_HIP else cand
-    seq = nq * nq > 64 and mc == len(mfma) - 1 and len(mfma) > 1 and mfma[mc] == 1
+    seq = mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or mfma[mc] == 1)
     prec = "PrecF64" if evaltype == EVAL_HIP else "PrecMixed"
     return f"stage_mfma<{prec}, {nq}, {mfma[mc]}, {'true' if seq else 'false'}>"
 
@@ -368,8 +368,8 @@ class Solver:
         """Time every tile shape of this solver's form and order on its mesh and
         keep the fastest for this solver."""
         best, n = C.c_int(0), C.c_int(0)
-        us = np.zeros(4)
-        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 4, C.byref(n)))
+        us = np.zeros(5)
+        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 5, C.byref(n)))
         return dict(best=best.value, us_per_launch=us[:n.value].tolist())
 
 

@@ -142,13 +142,15 @@ def test_serve_request_stream_ranges():
 
 
 def test_tile_shapes_host_side():
-    assert dgk.tile_shapes("HIP_MP", 7) == [1, 2, 4]
-    assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16]
+    # p <= 7: one, two, four wavefronts of elements, then one wavefront
+    # direction by direction
+    assert dgk.tile_shapes("HIP_MP", 7) == [1, 2, 4, 1]
+    assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16, 4]
     # p >= 8: one element, two elements, one element direction by direction
     assert dgk.tile_shapes("HIP_MP", 15) == [1, 2, 1]
     # FP64: the VALU kernel, then the FP64 MFMA kernel's shapes (two elements
     # of p = 15 would need more than 160 KiB of LDS)
-    assert dgk.tile_shapes("HIP", 7) == [1, 1, 2, 4]
+    assert dgk.tile_shapes("HIP", 7) == [1, 1, 2, 4, 1]
     assert dgk.tile_shapes("HIP", 15) == [1, 1, 1]
     with pytest.raises(dgk.DgkError) as ei:
         dgk.tile_select("HIP", 15, 3)
