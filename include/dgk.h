@@ -71,9 +71,21 @@ int dgk_evaltype_id(const char *name, int *id);
 /* Nq = p+1, Np = Nq^3, Nfp = Nq^2, NfpTot = 6*Nfp (mod_mesh.f90:236-241). */
 int dgk_element_sizes(int p, int *Nq, int *Np, int *Nfp, int *NfpTot);
 /* Read operator_data/p{p}.dat from dir: x1D(Nq), D1D(Nq,Nq),
- * Lift_mat(Nq,Nq,Nq,6), column-major (element_init, mod_mesh.f90:253-261). */
+ * Lift_mat(Nq,Nq,Nq,6), column-major (element_init, mod_mesh.f90:253-261).
+ * Orders 1..19 are accepted; above p = 15, for which the reference ships no
+ * file, a missing file is replaced by dgk_make_operators. */
 int dgk_load_operators(const char *dir, int p, double *x1D, double *D1D,
                        double *Lift_mat);
+/* The same three arrays from their closed forms, for any p >= 1: x1D the
+ * Legendre-Gauss-Lobatto nodes; D1D(i,j) = l_j'(x_i); Lift_mat(:,:,:,f) the
+ * tensor product of the identity with l-/+(x) = sum_n (2n+1)/2 P_n(x)
+ * P_n(-/+1) along face f's normal.  For p <= 15 they equal the reference's
+ * files to the files' rounding.  No counterpart in the reference, which
+ * stops at p = 15 (mod_dg_optr_kernel_opt1.F90.erb:1). */
+int dgk_make_operators(int p, double *x1D, double *D1D, double *Lift_mat);
+/* Write dir/p{p}.dat in the reference's format (three records of
+ * ES25.16E3 values) from dgk_make_operators. */
+int dgk_write_operators(const char *dir, int p);
 
 /* ---- structured periodic mesh (mesh_setup, mod_mesh.f90:74-193) ------- */
 /* Ne = NeX*NeY*NeZ, Nhalo = 2(NeY*NeZ + NeX*NeZ + NeX*NeY), NeA = Ne + Nhalo,

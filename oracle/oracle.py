@@ -91,7 +91,7 @@ def fmask(Nq: int) -> np.ndarray:
 class Mesh:
     """The reference mesh arrays built by the oracle (mesh_setup restated)."""
 
-    def __init__(self, NeX, NeY, NeZ, p, dom=(1.0, 1.0, 1.0)):
+    def __init__(self, NeX, NeY, NeZ, p, dom=(1.0, 1.0, 1.0), operator_dir=None):
         self.NeX, self.NeY, self.NeZ, self.p = NeX, NeY, NeZ, p
         self.Nq = Nq = p + 1
         self.Np, self.Nfp, self.NfpTot = Nq ** 3, Nq ** 2, 6 * Nq ** 2
@@ -99,7 +99,8 @@ class Mesh:
         self.Nhalo = 2 * (NeY * NeZ + NeX * NeZ + NeX * NeY)
         self.NeA = self.Ne + self.Nhalo
         self.NhaloNode = self.Nfp * self.Nhalo
-        self.x1D, self.D, self.Lift = load_operators(p)
+        self.x1D, self.D, self.Lift = (load_operators(p) if operator_dir is None
+                                       else load_operators(p, operator_dir))
         Ne, NfpTot, Np = self.Ne, self.NfpTot, self.Np
         self.VMapM = np.zeros((Ne, NfpTot), np.int32)
         self.VMapP = np.zeros((Ne, NfpTot), np.int32)

@@ -929,7 +929,7 @@ int check_form_order(int evaltype, int p) {
   if (evaltype != DGK_EVAL_HIP && evaltype != DGK_EVAL_HIP_MP)
     return set_error(DGK_ERR_UNSUPPORTED, "tile shapes exist for HIP and HIP_MP only");
   if (p < kMinNq - 1 || p > kMaxNq - 1)
-    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1.." + std::to_string(kMaxNq - 1));
   return DGK_OK;
 }
 

@@ -33,7 +33,9 @@ void clear_error();
   } while (0)
 
 constexpr int kMinNq = 2;
-constexpr int kMaxNq = 16;
+constexpr int kMaxNq = 20;      // p <= 19: operators above p = 15 are generated
+constexpr int kMaxNqFile = 16;  // operator_data/p{p}.dat exist for p <= 15
+constexpr int kMaxNqValu = 16;  // the VALU FP64 kernels; above, MFMA kernels only
 
 // Per-element geometry record: Escale[3] (2/dx_d, constant per element,
 // mod_mesh.f90:147-149), Fscale[6] and the outward normal of each face
@@ -112,6 +114,10 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
                             MeshTables &t);
 int load_operators(const char *dir, int p, std::vector<double> &x1D,
                    std::vector<double> &D, std::vector<double> &Lift);
+// Closed-form operators of order p (operators.cpp) and the p{p}.dat writer.
+int make_operators(int p, std::vector<double> &x1D, std::vector<double> &D,
+                   std::vector<double> &Lift);
+int write_operators(const char *dir, int p);
 
 // One instance of a batch: a periodic box of NeX x NeY x NeZ elements.
 struct BatchInstance {

@@ -576,13 +576,21 @@ unsigned grid1d(int64_t n, int threads) {
 // 1e7 DOF: the VALU kernel up to p = 7 and at p = 10, 11; the
 // direction-sequential one-element MFMA kernel (the last candidate) at p = 8,
 // 9 and 12..15.
-constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2};
+// Above p = 15 there is no VALU kernel: the list holds the MFMA shapes only.
+constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2,
+                                         0, 0, 0, 0};
 // Selected FP64 tile shape per Nq (see stage_f64_shapes).
-int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2};
+int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2, 0, 0, 0, 0};
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
   const int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0 : a.tile >= 0 ? a.tile : g_f64_shape[Nq];
+  if (Nq > kMaxNqValu && Nq <= kMaxNq) {
+    if (a.flux_in || !a.ell)
+      return set_error(DGK_ERR_UNSUPPORTED, "above p = 15 the stage runs on the MFMA kernel "
+                                            "only: no given flux, separable Lift only");
+    return launch_tile_f64(Nq, cand, a, s);
+  }
   if (cand > 0 && !a.flux_in && a.ell) return launch_tile_f64(Nq, cand - 1, a, s);
   switch (Nq) {
   case 2: return launch_nq<2>(a, s);
@@ -600,12 +608,13 @@ int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   case 14: return launch_nq<14>(a, s);
   case 15: return launch_nq<15>(a, s);
   case 16: return launch_nq<16>(a, s);
-  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   }
 }
 
 int prepare_stage_f64(int Nq) {
   DGK_TRY(prepare_tile_f64(Nq));
+  if (Nq > kMaxNqValu) return DGK_OK;
   switch (Nq) {
   case 2: return prepare_nq<2>();
   case 3: return prepare_nq<3>();
@@ -622,17 +631,19 @@ int prepare_stage_f64(int Nq) {
   case 14: return prepare_nq<14>();
   case 15: return prepare_nq<15>();
   case 16: return prepare_nq<16>();
-  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   }
 }
 
 // FP64 tile shapes: candidate 0 is the VALU kernel above (stage_f64), 1.. are
-// the FP64 MFMA kernel's shapes (kernels_mfma.hip).  The MFMA kernel computes
+// the FP64 MFMA kernel's shapes (kernels_mfma.hip); above p = 15 only the
+// MFMA shapes.  The MFMA kernel computes
 // the flux itself and applies the separable Lift, so launches that pass a
 // flux (cal_dqdt) or need the stored Lift always take stage_f64 /
 // stage_f64_lift.
 int stage_f64_shapes(int Nq, int *epb, int n) {
   if (!epb || n < kMaxTileShapes) return 0;
+  if (Nq > kMaxNqValu) return tile_f64_shapes(Nq, epb, n - 1);
   switch (Nq) {
 #define DGK_F64_SHAPE(N) \
   case N: epb[0] = Staged<N>::EPB; break;
@@ -648,7 +659,7 @@ int stage_f64_shapes(int Nq, int *epb, int n) {
 int stage_f64_select(int Nq, int cand) {
   int e[kMaxTileShapes];
   const int nc = stage_f64_shapes(Nq, e, kMaxTileShapes);
-  if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   if (cand == -1) cand = kF64Default[Nq];
   if (cand < 0 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));

@@ -94,6 +94,7 @@ struct MP {
   static constexpr int WAVES = THREADS / 64;
   static constexpr int ROWS = ((LANES + 15) / 16) * 16;     // pencils per direction
   static constexpr int MTILES = ROWS / 16;
+  static constexpr int NT = (NQ + 15) / 16;                // 16-wide column tiles of B
   static constexpr int KAUG = NQ + 2;
   static constexpr int KSTEPS = (KAUG + 3) / 4;
   static constexpr int KPAD = 4 * KSTEPS;
@@ -127,6 +128,7 @@ constexpr bool kind_seq(int k) {
 template <class P, int NQ>
 constexpr bool kind_ok(int k) {
   if (NQ * NQ > 64 && k == 3) return false;
+  if (NQ > kMaxNqValu && k != 2) return false;  // above p = 15: direction by direction only
   return k == 0   ? MP<P, NQ, kind_epb<NQ>(0), kind_seq<NQ>(0)>::LDS_BYTES <= 160 * 1024
          : k == 1 ? MP<P, NQ, kind_epb<NQ>(1), kind_seq<NQ>(1)>::LDS_BYTES <= 160 * 1024
          : k == 2 ? MP<P, NQ, kind_epb<NQ>(2), kind_seq<NQ>(2)>::LDS_BYTES <= 160 * 1024
@@ -149,18 +151,29 @@ constexpr int kind_of(int c) {
   return -1;
 }
 
-// One 16-pencil tile: A rows from `tile`, B in registers; the result
-// overwrites the tile's first NQ columns.
-template <class P, int KSTEPS, int KPAD, int NQ>
+// One 16-pencil tile: A rows from `tile`, B (NT column tiles) in registers;
+// all A values are read before any result overwrites the tile's first NQ
+// columns.
+template <class P, int KSTEPS, int KPAD, int NQ, int NT>
 __device__ __forceinline__ void mfma_tile(typename P::op_t *tile,
-                                          const typename P::op_t (&bop)[KSTEPS], int rd,
+                                          const typename P::op_t (&bop)[NT][KSTEPS], int rd,
                                           int wr, int col) {
-  typename P::acc_t acc = {0, 0, 0, 0};
+  typename P::op_t a[KSTEPS];
 #pragma unroll
-  for (int ks = 0; ks < KSTEPS; ++ks) acc = P::mfma(tile[rd + ks], bop[ks], acc);
-  if (col < NQ) {
+  for (int ks = 0; ks < KSTEPS; ++ks) a[ks] = tile[rd + ks];
+  typename P::acc_t acc[NT];
 #pragma unroll
-    for (int t4 = 0; t4 < 4; ++t4) tile[wr + t4 * P::RSTRIDE_T * KPAD] = acc[t4];
+  for (int nt = 0; nt < NT; ++nt) {
+    acc[nt] = typename P::acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) acc[nt] = P::mfma(a[ks], bop[nt][ks], acc[nt]);
+  }
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    if (16 * nt + col < NQ) {
+#pragma unroll
+      for (int t4 = 0; t4 < 4; ++t4) tile[wr + 16 * nt + t4 * P::RSTRIDE_T * KPAD] = acc[nt][t4];
+    }
   }
 }
 
@@ -273,13 +286,18 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   }
 
   // B = [D^T ; l- ; l+ ; 0] rows of this lane (k permuted, see above)
+  constexpr int NT = C::NT;
   const int col = lane & 15, kq = lane >> 4;
-  op_t bop[KSTEPS];
+  op_t bop[NT][KSTEPS];
 #pragma unroll
-  for (int ks = 0; ks < KSTEPS; ++ks) {
-    const int l = kq * KSTEPS + ks;
-    bop[ks] = col >= NQ ? (op_t)0 : l < NQ ? D_[col + l * NQ] : l == NQ ? ell_[col]
-              : l == NQ + 1 ? ell_[NQ + col] : (op_t)0;
+  for (int nt = 0; nt < NT; ++nt) {
+    const int c = 16 * nt + col;
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+      const int l = kq * KSTEPS + ks;
+      bop[nt][ks] = c >= NQ ? (op_t)0 : l < NQ ? D_[c + l * NQ] : l == NQ ? ell_[c]
+                    : l == NQ + 1 ? ell_[NQ + c] : (op_t)0;
+    }
   }
   const int rd = col * KPAD + kq * KSTEPS;        // A read offset of this lane in a tile
   const int wr = P::RSTRIDE_Q * kq * KPAD + col;  // result offset of value 0
@@ -328,7 +346,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       const int job = wave + jj * C::WAVES;
       if (job >= C::JOBS) break;  // wave-uniform
       const int d = job / C::MTILES, t = job - d * C::MTILES;
-      mfma_tile<P, KSTEPS, KPAD, NQ>(As + d * AREA + t * 16 * KPAD, bop, rd, wr, col);
+      mfma_tile<P, KSTEPS, KPAD, NQ, NT>(As + d * AREA + t * 16 * KPAD, bop, rd, wr, col);
     }
     __syncthreads();
 
@@ -366,7 +384,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       for (int jj = 0; jj < C::JPW; ++jj) {
         const int t = wave + jj * C::WAVES;
         if (t >= C::MTILES) break;  // wave-uniform
-        mfma_tile<P, KSTEPS, KPAD, NQ>(As + t * 16 * KPAD, bop, rd, wr, col);
+        mfma_tile<P, KSTEPS, KPAD, NQ, NT>(As + t * 16 * KPAD, bop, rd, wr, col);
       }
       __syncthreads();
       if (active) {
@@ -478,7 +496,9 @@ int shapes_of(int *epb) {
   case 8: return CALL(8); case 9: return CALL(9); case 10: return CALL(10);              \
   case 11: return CALL(11); case 12: return CALL(12); case 13: return CALL(13);          \
   case 14: return CALL(14); case 15: return CALL(15); case 16: return CALL(16);          \
-  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");                   \
+  case 17: return CALL(17); case 18: return CALL(18); case 19: return CALL(19);          \
+  case 20: return CALL(20);                                                              \
+  default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));                   \
   }
 
 int prepare_stage_mp(int Nq) {
@@ -524,7 +544,7 @@ int tile_f64_shapes(int Nq, int *epb, int n) {
 int stage_mp_select(int Nq, int cand) {
   int e[kKinds];
   const int nc = stage_mp_shapes(Nq, e, kKinds);
-  if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16");
+  if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   if (cand == -1) cand = 0;  // the default
   if (cand < 0 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));

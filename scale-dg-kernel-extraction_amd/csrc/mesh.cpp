@@ -112,11 +112,13 @@ int load_operators(const char *dir, int p, std::vector<double> &x1D,
                    std::vector<double> &D, std::vector<double> &Lift) {
   if (!dir) return set_error(DGK_ERR_ARG, "operator directory is NULL");
   if (p < kMinNq - 1 || p > kMaxNq - 1)
-    return set_error(DGK_ERR_UNSUPPORTED,
-                     "polynomial order " + std::to_string(p) + " outside 1..15");
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order " + std::to_string(p) +
+                                              " outside 1.." + std::to_string(kMaxNq - 1));
   const int Nq = p + 1;
   const std::string path = std::string(dir) + "/p" + std::to_string(p) + ".dat";
   std::ifstream in(path);
+  // orders beyond the stored files use the closed forms
+  if (!in && Nq > kMaxNqFile) return make_operators(p, x1D, D, Lift);
   if (!in) return set_error(DGK_ERR_IO, "cannot open operator file " + path);
   x1D.assign(Nq, 0.0);
   D.assign((size_t)Nq * Nq, 0.0);
@@ -135,7 +137,7 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
   if (NeX < 1 || NeY < 1 || NeZ < 1)
     return set_error(DGK_ERR_ARG, "element counts must be >= 1");
   if (p < kMinNq - 1 || p > kMaxNq - 1)
-    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1.." + std::to_string(kMaxNq - 1));
   if (!(dom_x > 0 && dom_y > 0 && dom_z > 0))
     return set_error(DGK_ERR_ARG, "domain sizes must be positive");
   const int Nq = p + 1;
@@ -183,7 +185,7 @@ int build_tables_batch(int p, const double *D1D, const double *Lift_mat,
                        const std::vector<BatchInstance> &inst, MeshTables &t,
                        std::vector<int64_t> &elem_offset) {
   if (p < kMinNq - 1 || p > kMaxNq - 1)
-    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1.." + std::to_string(kMaxNq - 1));
   if (inst.empty()) return set_error(DGK_ERR_ARG, "empty batch");
   const int Nq = p + 1;
   t.Nq = Nq; t.Np = Nq * Nq * Nq; t.Nfp = Nq * Nq; t.NfpTot = 6 * t.Nfp;
@@ -241,7 +243,7 @@ int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
       (NhaloNode > 0 && !halo_src_map))
     return set_error(DGK_ERR_ARG, "null array argument");
   if (Nq < kMinNq || Nq > kMaxNq)
-    return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2..16 (p 1..15)");
+    return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   if (Ne < 1 || NeA < Ne || NhaloNode < 0)
     return set_error(DGK_ERR_ARG, "inconsistent Ne / NeA / NhaloNode");
   const int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NfpTot = 6 * Nfp;
@@ -373,7 +375,7 @@ extern "C" int dgk_load_operators(const char *dir, int p, double *x1D,
 extern "C" int dgk_element_sizes(int p, int *Nq, int *Np, int *Nfp, int *NfpTot) {
   clear_error();
   if (p < kMinNq - 1 || p > kMaxNq - 1)
-    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1.." + std::to_string(kMaxNq - 1));
   const int q = p + 1;
   if (Nq) *Nq = q;
   if (Np) *Np = q * q * q;
@@ -387,7 +389,7 @@ extern "C" int dgk_mesh_sizes(int NeX, int NeY, int NeZ, int p, int64_t *Ne,
   clear_error();
   if (NeX < 1 || NeY < 1 || NeZ < 1) return set_error(DGK_ERR_ARG, "element counts must be >= 1");
   if (p < kMinNq - 1 || p > kMaxNq - 1)
-    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1..15");
+    return set_error(DGK_ERR_UNSUPPORTED, "polynomial order outside 1.." + std::to_string(kMaxNq - 1));
   const int64_t ne = (int64_t)NeX * NeY * NeZ;
   const int64_t nh = 2 * ((int64_t)NeY * NeZ + (int64_t)NeX * NeZ + (int64_t)NeX * NeY);
   if (Ne) *Ne = ne;

@@ -572,7 +572,7 @@ int check_request(const dgk_request *r) {
   }
   if (r->p < kMinNq - 1 || r->p > kMaxNq - 1)
     return set_error(DGK_ERR_UNSUPPORTED, "polynomial order " + std::to_string(r->p) +
-                                              " outside 1..15");
+                                              " outside 1.." + std::to_string(kMaxNq - 1));
   if (r->nstep < 0) return set_error(DGK_ERR_ARG, "nstep must be >= 0");
   if (r->tol_class != DGK_TOL_FP64 && r->tol_class != DGK_TOL_1E5)
     return set_error(DGK_ERR_ARG, "unknown tolerance class " + std::to_string(r->tol_class));
@@ -594,7 +594,7 @@ int dgk_serve_footprint(const int32_t ne[3], int p, int tol_class, int64_t *byte
   clear_error();
   if (!ne || !bytes) return set_error(DGK_ERR_ARG, "NULL argument");
   if (ne[0] < 1 || ne[1] < 1 || ne[2] < 1) return set_error(DGK_ERR_ARG, "elements per axis must be >= 1");
-  if (p < kMinNq - 1 || p > kMaxNq - 1) return set_error(DGK_ERR_UNSUPPORTED, "order outside 1..15");
+  if (p < kMinNq - 1 || p > kMaxNq - 1) return set_error(DGK_ERR_UNSUPPORTED, "order outside 1.." + std::to_string(kMaxNq - 1));
   const int Np = (p + 1) * (p + 1) * (p + 1);
   *bytes = layout((int64_t)ne[0] * ne[1] * ne[2], Np, tol_class == DGK_TOL_1E5).total;
   return DGK_OK;

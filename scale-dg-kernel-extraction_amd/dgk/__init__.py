@@ -40,6 +40,8 @@ _SIGNATURES = {
     "dgk_evaltype_id": (_I, [_S, _P]),
     "dgk_element_sizes": (_I, [_I, _P, _P, _P, _P]),
     "dgk_load_operators": (_I, [_S, _I, _P, _P, _P]),
+    "dgk_make_operators": (_I, [_I, _P, _P, _P]),
+    "dgk_write_operators": (_I, [_S, _I]),
     "dgk_mesh_sizes": (_I, [_I, _I, _I, _I, _P, _P, _P]),
     "dgk_mesh_build": (_I, [_I, _I, _I, _I, _P, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P]),
     "dgk_create": (_I, [_I, _I, _L, _L, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -158,12 +160,13 @@ def kernel_name(evaltype, p: int, cand: int) -> str:
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
     nq = p + 1
-    if evaltype == EVAL_HIP and cand == 0:
+    valu = evaltype == EVAL_HIP and nq <= 16  # the FP64 VALU kernel is candidate 0
+    if valu and cand == 0:
         return f"stage_f64<{nq}>"
     shapes = tile_shapes(evaltype, p)
-    mfma = shapes[1:] if evaltype == EVAL_HIP else shapes
-    mc = cand - 1 if evaltype == EVAL_HIP else cand
-    seq = mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or mfma[mc] == 1)
+    mfma = shapes[1:] if valu else shapes
+    mc = cand - 1 if valu else cand
+    seq = nq > 16 or (mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or mfma[mc] == 1))
     prec = "PrecF64" if evaltype == EVAL_HIP else "PrecMixed"
     return f"stage_mfma<{prec}, {nq}, {mfma[mc]}, {'true' if seq else 'false'}>"
 
@@ -197,6 +200,21 @@ def load_operators(p: int, directory: str | os.PathLike = OPERATOR_DIR):
     L = np.zeros((6, Nq, Nq, Nq))
     _check(lib().dgk_load_operators(str(directory).encode(), p, _ptr(x), _ptr(D), _ptr(L)))
     return x, D, L
+
+
+def make_operators(p: int):
+    """Closed-form x1D, D1D and Lift_mat of order p, laid out as load_operators."""
+    Nq = p + 1
+    x = np.zeros(Nq)
+    D = np.zeros((Nq, Nq))
+    L = np.zeros((6, Nq, Nq, Nq))
+    _check(lib().dgk_make_operators(int(p), _ptr(x), _ptr(D), _ptr(L)))
+    return x, D, L
+
+
+def write_operators(directory, p: int) -> None:
+    """Write directory/p{p}.dat in the reference's operator format."""
+    _check(lib().dgk_write_operators(str(directory).encode(), int(p)))
 
 
 def mesh_sizes(NeX: int, NeY: int, NeZ: int, p: int) -> dict:

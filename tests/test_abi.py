@@ -48,7 +48,7 @@ def test_sizes_and_operator_errors():
     assert dgk.element_sizes(7) == dict(Nq=8, Np=512, Nfp=64, NfpTot=384)
     assert dgk.mesh_sizes(32, 32, 32, 7) == dict(Ne=32768, NeA=38912, NhaloNode=393216)
     with pytest.raises(dgk.DgkError) as ei:
-        dgk.element_sizes(16)
+        dgk.element_sizes(20)
     assert ei.value.code == dgk.ERR_UNSUPPORTED
     with pytest.raises(dgk.DgkError) as ei:
         dgk.load_operators(3, "/nonexistent")
@@ -118,7 +118,7 @@ def test_serve_footprint_and_no_device():
         assert got % 65536 == 0
         assert fields + 24 * ne_tot <= got <= fields + 24 * ne_tot + 8 * 256 + 8224 + 65536 + 256 * 10
     with pytest.raises(dgk.DgkError) as ei:
-        serving.footprint((4, 4, 4), 16, 0)
+        serving.footprint((4, 4, 4), 20, 0)
     assert ei.value.code == dgk.ERR_UNSUPPORTED
     if dgk.device_count() == 0:
         with pytest.raises(dgk.DgkError) as ei:
@@ -172,3 +172,16 @@ def test_tile_shapes_host_side():
     assert dgk.tile_selected("HIP", 13) == 0
     dgk.tile_select("HIP", 13, -1)
     assert dgk.tile_selected("HIP", 13) == 3
+
+
+def test_shapes_above_the_operator_files():
+    # p = 16..19 run on the direction-by-direction MFMA kernels only
+    for p in (16, 19):
+        assert dgk.tile_shapes("HIP", p) == [1]
+        assert dgk.tile_shapes("HIP_MP", p) == [1]
+        assert dgk.tile_selected("HIP", p) == 0
+    assert dgk.kernel_name("HIP", 17, 0) == "stage_mfma<PrecF64, 18, 1, true>"
+    assert dgk.kernel_name("HIP", 7, 0) == "stage_f64<8>"
+    assert dgk.kernel_name("HIP", 7, 4) == "stage_mfma<PrecF64, 8, 1, true>"
+    assert dgk.kernel_name("HIP_MP", 15, 2) == "stage_mfma<PrecMixed, 16, 1, true>"
+    assert dgk.element_sizes(19) == dict(Nq=20, Np=8000, Nfp=400, NfpTot=2400)

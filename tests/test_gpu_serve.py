@@ -143,7 +143,7 @@ def test_refusals():
             sv.submit(big)
         assert ei.value.code == dgk.ERR_NOMEM
         ok = dict(ne=(2, 2, 2), p=2, nstep=2, vel=(1, 0, 0), centre=(0.5,) * 3, sigma=0.1)
-        for bad, code in ((dict(ok, p=16), dgk.ERR_UNSUPPORTED), (dict(ok, sigma=0.0), dgk.ERR_ARG),
+        for bad, code in ((dict(ok, p=20), dgk.ERR_UNSUPPORTED), (dict(ok, sigma=0.0), dgk.ERR_ARG),
                           (dict(ok, ne=(0, 2, 2)), dgk.ERR_ARG),
                           (dict(ok, vel=(0, 0, 0)), dgk.ERR_ARG),
                           (dict(ok, tol_class=7), dgk.ERR_ARG)):
