@@ -226,11 +226,13 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  *           elements for p <= 7; for p >= 8 one element, two elements, and
  *           one element processed direction by direction (a third of the
  *           LDS).  Candidate 0 is the default.
- *   HIP:    candidate 0 is the VALU kernel; candidates 1.. are the FP64
- *           MFMA kernel's shapes as for HIP_MP, limited to 160 KiB of LDS.
- *           The default is the VALU kernel for p <= 7 and p = 10, 11, and the
- *           direction-by-direction MFMA shape (the last candidate) for p = 8,
- *           9 and 12..15.
+ *   HIP:    candidate 0 is the VALU kernel with q, u, v, w staged in LDS,
+ *           1 the VALU kernel with v, w in registers (half the LDS);
+ *           candidates 2.. are the FP64 MFMA kernel's shapes as for HIP_MP,
+ *           limited to 160 KiB of LDS; above p = 15 only the MFMA shapes.
+ *           Defaults: the register VALU kernel at p = 3, 6, 7, 9..13, the
+ *           direction-by-direction MFMA shape (the last candidate) at
+ *           p = 8, 14, 15, the staged VALU kernel otherwise.
  * dgk_tile_select sets the process-wide choice for that form and order
  * (-1 restores the default); it holds for every solver that has not been
  * autotuned.  Launches with a given flux (dgk_cal_dqdt) or a stored,
@@ -244,9 +246,10 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  * chosen candidate, its time and the default's time (the batch's captured
  * step is re-captured at the next dgk_batch_step).  No counterpart
  * in the reference, whose OPT1 kernels are generated per order
- * (mod_dg_optr_kernel_opt1.F90.erb).  The MFMA shapes of one form give the
- * same bits as each other; the FP64 VALU and MFMA kernels differ by
- * rounding only, both within the FP64 parity bar. */
+ * (mod_dg_optr_kernel_opt1.F90.erb).  The two FP64 VALU kernels give the
+ * same bits, and so do the MFMA shapes of one form among themselves; the
+ * FP64 VALU and MFMA kernels differ by rounding only, both within the FP64
+ * parity bar. */
 int dgk_tile_shapes(int evaltype, int p, int *epb, int max, int *n);
 int dgk_tile_select(int evaltype, int p, int cand);
 int dgk_tile_selected(int evaltype, int p, int *cand);

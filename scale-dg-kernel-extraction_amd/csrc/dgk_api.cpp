@@ -967,7 +967,7 @@ int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
   int epb[kMaxTileShapes];
   const int nc = tile_shapes(s->evaltype, s->Nq, epb);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "no tile shapes for this form and order");
-  double t[kMaxTileShapes] = {0, 0, 0, 0, 0};
+  double t[kMaxTileShapes] = {};
   int arg = 0;
   for (int c = 0; c < nc; ++c) {
     s->tile = c;
@@ -1039,7 +1039,7 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
   for (dgk_solver *s : b->groups) DGK_HIP(hipStreamSynchronize(s->stream));
   for (size_t g = 0; g < b->groups.size(); ++g) {
     dgk_solver *s = b->groups[g];
-    double t[kMaxTileShapes] = {0, 0, 0, 0, 0};
+    double t[kMaxTileShapes] = {};
     int arg = 0, nc = 0;
     DGK_TRY(autotune(s, reps, &arg, t, kMaxTileShapes, &nc));
     if ((int)g < max) {

@@ -200,8 +200,8 @@ int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
 int stage_mp_shapes(int Nq, int *epb, int n);
 int stage_mp_select(int Nq, int cand);
 int stage_mp_selected(int Nq);
-// FP64: candidate 0 is the VALU kernel, 1.. the FP64 MFMA kernel's shapes.
-constexpr int kMaxTileShapes = 5;
+// FP64: candidates 0, 1 are the VALU kernels, 2.. the FP64 MFMA kernel's shapes.
+constexpr int kMaxTileShapes = 6;
 int stage_f64_shapes(int Nq, int *epb, int n);
 int stage_f64_select(int Nq, int cand);
 int stage_f64_selected(int Nq);

@@ -18,6 +18,10 @@
 //      on all six faces into registers;
 //   2. upwind flux of face node (i,j) of every face from the LDS copy;
 //   3. F = q*(u,v,w) in place in LDS;
+//   With VREG only q and u are staged: the lane loads its v and w columns
+//   (and the v traces of its y faces) into registers, and Fy takes the place
+//   of q.  That halves the LDS per element (more workgroups per CU) for a
+//   few more registers; the arithmetic is the same.
 //   4. per node: the three sum-factorised contractions from LDS (D rows in
 //      registers), the Lift in tensor-product form -- face f contributes
 //      l-/+(x_n) * flux_f(tangential node) along its normal n, with l-/+ taken
@@ -50,8 +54,9 @@ __device__ __forceinline__ int64_t uniform(int64_t v) {
   return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-template <int NQ>
+template <int NQ, bool VREG_>
 struct Staged {
+  static constexpr bool VREG = VREG_;
   static constexpr int NQ2 = NQ * NQ;
   static constexpr int NP = NQ2 * NQ;
   static constexpr int EPB = NQ2 <= 64 ? 64 / NQ2 : 1;  // elements per workgroup
@@ -63,7 +68,8 @@ struct Staged {
   static constexpr int CHUNK = VEC16 ? 128 : 32;  // doubles per wave instruction
   static constexpr int NCHUNK = (EPB * NP + CHUNK - 1) / CHUNK;
   static constexpr int FIELD = NCHUNK * CHUNK;    // padded staged field (doubles)
-  static constexpr int OFF_G = 4 * FIELD;         // x/y-face fluxes [EPB][4][NQ2]
+  static constexpr int NSTAGED = VREG ? 2 : 4;     // staged fields: q, u (, v, w)
+  static constexpr int OFF_G = NSTAGED * FIELD;   // x/y-face fluxes [EPB][4][NQ2]
   static constexpr int OFF_D = OFF_G + EPB * 4 * NQ2;
   static constexpr int OFF_L = OFF_D + NQ2;
   static constexpr int LDS_DOUBLES = OFF_L + 2 * NQ;
@@ -74,8 +80,8 @@ struct Staged {
   static constexpr bool FX_T = WAVES == 1;
 };
 
-template <int NQ>
-__global__ void __launch_bounds__(Staged<NQ>::THREADS)
+template <int NQ, bool VREG>
+__global__ void __launch_bounds__((Staged<NQ, VREG>::THREADS))
 stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
           const double *__restrict__ v_, const double *__restrict__ w_, double *out_,
           const double *__restrict__ flux_in_, const int32_t *__restrict__ nbr_,
@@ -86,9 +92,10 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
           const int mode_, const int axis_aligned_, const int dt_geo_) {
   // Read-only operands are __restrict__ kernel parameters, so wave-uniform
   // reads of them (operators, geometry, neighbour table) are scalar loads.
-  using C = Staged<NQ>;
+  using C = Staged<NQ, VREG>;
   constexpr int NQ2 = C::NQ2, NP = C::NP, EPB = C::EPB, NFT = 6 * NQ2;
   extern __shared__ double smem[];
+  // VREG: sv, sw are not staged (the pointers are unused)
   double *sq = smem, *su = smem + C::FIELD, *sv = smem + 2 * C::FIELD,
          *sw = smem + 3 * C::FIELD;
   double *sG = smem + C::OFF_G, *sD = smem + C::OFF_D, *sL = smem + C::OFF_L;
@@ -110,7 +117,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
         int64_t off = 0;
         if (del < EPB && sl < Ne_) off = (order_ ? (int64_t)order_[sl] : sl) * NP + dn;
 #pragma unroll
-        for (int fld = 0; fld < 4; ++fld) {
+        for (int fld = 0; fld < C::NSTAGED; ++fld) {
           const char *gp = (const char *)(src[fld] + off) + (C::VEC16 ? 0 : 4 * (lane & 1));
           __attribute__((address_space(3))) void *lp =
               (__attribute__((address_space(3))) void *)(smem + fld * C::FIELD + c * C::CHUNK);
@@ -179,10 +186,21 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
       }
     }
   }
+  // VREG: the v and w columns and the v traces of the y faces
+  double vc[VREG ? NQ : 1], wc[VREG ? NQ : 1], vM[2];
+  if (VREG && active) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      vc[k] = v_[base + (int64_t)k * NQ2];
+      wc[k] = w_[base + (int64_t)k * NQ2];
+    }
+    vM[0] = v_[e * NP + own_face_node(0, i, j, NQ)];
+    vM[1] = v_[e * NP + own_face_node(2, i, j, NQ)];
+  }
+  // the RK base and the output are touched once per launch: non-temporal,
+  // so L2 keeps the neighbour traces
   if (active && use_q0) {
 #pragma unroll
-    // the RK base and the output are touched once per launch: non-temporal,
-    // so L2 keeps the neighbour traces
     for (int k = 0; k < NQ; ++k) q0v[k] = __builtin_nontemporal_load(q0_ + base + (int64_t)k * NQ2);
   }
   __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
@@ -202,12 +220,22 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
         const double qM = sq[m];
         double VelM, VelP;
         if (axal) {
-          const double *sa = kFaceAxis[f] == 0 ? su : kFaceAxis[f] == 1 ? sv : sw;
+          double vm;
+          if constexpr (VREG)
+            vm = kFaceAxis[f] == 0 ? su[m] : f == 0 ? vM[0] : f == 2 ? vM[1]
+                 : f == 4 ? wc[0] : wc[NQ - 1];
+          else
+            vm = (kFaceAxis[f] == 0 ? su : kFaceAxis[f] == 1 ? sv : sw)[m];
           const double na = g[kGeoNormal + 3 * f + kFaceAxis[f]];
-          VelM = sa[m] * na;
+          VelM = vm * na;
           VelP = uP[f] * na;
         } else {
-          VelM = su[m] * n1 + sv[m] * n2 + sw[m] * n3;
+          if constexpr (VREG) {
+            const int64_t gm = e * NP + own_face_node(f, i, j, NQ);
+            VelM = su[m] * n1 + v_[gm] * n2 + w_[gm] * n3;
+          } else {
+            VelM = su[m] * n1 + sv[m] * n2 + sw[m] * n3;
+          }
           VelP = uP[f] * n1 + vP[f] * n2 + wP[f] * n3;
         }
         const double alpha = 0.5 * fabs(VelP + VelM);
@@ -225,12 +253,13 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const int n = lb + ij + k * NQ2;
-      const double qq = sq[n], uu = su[n], vv = sv[n], ww = sw[n];
+      const double qq = sq[n], uu = su[n];
+      const double vv = VREG ? vc[k] : sv[n], ww = VREG ? wc[k] : sw[n];
       qc[k] = qq;
       fz[k] = qq * ww;
       if (C::FX_T) su[lb + j + i * NQ + k * NQ2] = qq * uu;
       else su[n] = qq * uu;
-      sv[n] = qq * vv;
+      (VREG ? sq : sv)[n] = qq * vv;  // Fy (in place of q with VREG)
     }
   }
   __syncthreads();
@@ -252,7 +281,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
         dx += Di[l] * (C::FX_T ? su[lb + j + l * NQ + k * NQ2] : su[lb + l + j * NQ + k * NQ2]);
-        dy += Dj[l] * sv[lb + i + l * NQ + k * NQ2];
+        dy += Dj[l] * (VREG ? sq : sv)[lb + i + l * NQ + k * NQ2];
         dz += sD[k + l * NQ] * fz[l];  // wave-uniform LDS broadcast
       }
       const double lift = lmj * G[0 * NQ2 + i + k * NQ] + lpi * G[1 * NQ2 + j + k * NQ] +
@@ -405,28 +434,43 @@ stage_f64_lift(const StageArgs P) {
   }
 }
 
-template <int NQ>
-int prepare_nq() {
+template <int NQ, bool VREG>
+int prepare_shape_valu() {
   static int prepared_dev = -1;
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)Staged<NQ>::LDS_BYTES));
+                                (int)Staged<NQ, VREG>::LDS_BYTES));
     prepared_dev = dev;
   }
   return DGK_OK;
 }
 
 template <int NQ>
-int launch_nq(const StageArgs &a, hipStream_t s) {
-  if (a.ell) {
-    using T = Staged<NQ>;
+int prepare_nq() {
+  DGK_TRY((prepare_shape_valu<NQ, false>()));
+  return prepare_shape_valu<NQ, true>();
+}
+
+// VALU stage kernel (VREG: v, w in registers), or the stored-Lift kernel.
+template <int NQ>
+int launch_nq(const StageArgs &a, hipStream_t s, bool vreg) {
+  if (a.ell && vreg) {
+    using T = Staged<NQ, true>;
     const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
     if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-    DGK_TRY(prepare_nq<NQ>());
-    hipLaunchKernelGGL(stage_f64<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
+    DGK_TRY((prepare_shape_valu<NQ, true>()));
+    hipLaunchKernelGGL((stage_f64<NQ, true>), dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
+                       a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
+                       a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo);
+  } else if (a.ell) {
+    using T = Staged<NQ, false>;
+    const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
+    if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
+    DGK_TRY((prepare_shape_valu<NQ, false>()));
+    hipLaunchKernelGGL((stage_f64<NQ, false>), dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
                        a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
                        a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo);
   } else {
@@ -572,15 +616,20 @@ unsigned grid1d(int64_t n, int threads) {
 
 }  // namespace
 
-// Default FP64 tile shape per Nq, from per-order timings on MI355X at about
-// 1e7 DOF: the VALU kernel up to p = 7 and at p = 10, 11; the
-// direction-sequential one-element MFMA kernel (the last candidate) at p = 8,
-// 9 and 12..15.
-// Above p = 15 there is no VALU kernel: the list holds the MFMA shapes only.
-constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2,
+// FP64 tile shapes: candidate 0 is the VALU kernel with q, u, v, w staged in
+// LDS, 1 the VALU kernel with v, w in registers (VREG), 2.. the FP64 MFMA
+// kernel's shapes (kernels_mfma.hip); above p = 15 only the MFMA shapes.  The
+// MFMA kernel computes the flux itself and applies the separable Lift, so
+// launches that pass a flux (cal_dqdt) or need the stored Lift take the VALU
+// kernels (stage_f64 / stage_f64_lift).
+constexpr int kValuShapes = 2;
+// Default per Nq, from per-order timings on MI355X at 2e6..1.7e7 DOF: the
+// register VALU kernel at p = 3, 6, 7, 9..13, the direction-by-direction MFMA
+// kernel (the last candidate) at p = 8, 14, 15, the staged VALU kernel else.
+constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 4, 1, 1, 1, 1, 1, 3, 3,
                                          0, 0, 0, 0};
 // Selected FP64 tile shape per Nq (see stage_f64_shapes).
-int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0, 0, 3, 3, 2, 2, 0, 0, 0, 0};
+int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 4, 1, 1, 1, 1, 1, 3, 3, 0, 0, 0, 0};
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
@@ -591,23 +640,16 @@ int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
                                             "only: no given flux, separable Lift only");
     return launch_tile_f64(Nq, cand, a, s);
   }
-  if (cand > 0 && !a.flux_in && a.ell) return launch_tile_f64(Nq, cand - 1, a, s);
+  if (cand >= kValuShapes && !a.flux_in && a.ell)
+    return launch_tile_f64(Nq, cand - kValuShapes, a, s);
+  const bool vreg = cand == 1;
   switch (Nq) {
-  case 2: return launch_nq<2>(a, s);
-  case 3: return launch_nq<3>(a, s);
-  case 4: return launch_nq<4>(a, s);
-  case 5: return launch_nq<5>(a, s);
-  case 6: return launch_nq<6>(a, s);
-  case 7: return launch_nq<7>(a, s);
-  case 8: return launch_nq<8>(a, s);
-  case 9: return launch_nq<9>(a, s);
-  case 10: return launch_nq<10>(a, s);
-  case 11: return launch_nq<11>(a, s);
-  case 12: return launch_nq<12>(a, s);
-  case 13: return launch_nq<13>(a, s);
-  case 14: return launch_nq<14>(a, s);
-  case 15: return launch_nq<15>(a, s);
-  case 16: return launch_nq<16>(a, s);
+#define DGK_VALU(N) \
+  case N: return launch_nq<N>(a, s, vreg);
+  DGK_VALU(2) DGK_VALU(3) DGK_VALU(4) DGK_VALU(5) DGK_VALU(6) DGK_VALU(7) DGK_VALU(8)
+  DGK_VALU(9) DGK_VALU(10) DGK_VALU(11) DGK_VALU(12) DGK_VALU(13) DGK_VALU(14) DGK_VALU(15)
+  DGK_VALU(16)
+#undef DGK_VALU
   default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   }
 }
@@ -616,44 +658,29 @@ int prepare_stage_f64(int Nq) {
   DGK_TRY(prepare_tile_f64(Nq));
   if (Nq > kMaxNqValu) return DGK_OK;
   switch (Nq) {
-  case 2: return prepare_nq<2>();
-  case 3: return prepare_nq<3>();
-  case 4: return prepare_nq<4>();
-  case 5: return prepare_nq<5>();
-  case 6: return prepare_nq<6>();
-  case 7: return prepare_nq<7>();
-  case 8: return prepare_nq<8>();
-  case 9: return prepare_nq<9>();
-  case 10: return prepare_nq<10>();
-  case 11: return prepare_nq<11>();
-  case 12: return prepare_nq<12>();
-  case 13: return prepare_nq<13>();
-  case 14: return prepare_nq<14>();
-  case 15: return prepare_nq<15>();
-  case 16: return prepare_nq<16>();
+#define DGK_VALU(N) \
+  case N: return prepare_nq<N>();
+  DGK_VALU(2) DGK_VALU(3) DGK_VALU(4) DGK_VALU(5) DGK_VALU(6) DGK_VALU(7) DGK_VALU(8)
+  DGK_VALU(9) DGK_VALU(10) DGK_VALU(11) DGK_VALU(12) DGK_VALU(13) DGK_VALU(14) DGK_VALU(15)
+  DGK_VALU(16)
+#undef DGK_VALU
   default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   }
 }
 
-// FP64 tile shapes: candidate 0 is the VALU kernel above (stage_f64), 1.. are
-// the FP64 MFMA kernel's shapes (kernels_mfma.hip); above p = 15 only the
-// MFMA shapes.  The MFMA kernel computes
-// the flux itself and applies the separable Lift, so launches that pass a
-// flux (cal_dqdt) or need the stored Lift always take stage_f64 /
-// stage_f64_lift.
 int stage_f64_shapes(int Nq, int *epb, int n) {
   if (!epb || n < kMaxTileShapes) return 0;
-  if (Nq > kMaxNqValu) return tile_f64_shapes(Nq, epb, n - 1);
+  if (Nq > kMaxNqValu) return tile_f64_shapes(Nq, epb, n - kValuShapes);
   switch (Nq) {
 #define DGK_F64_SHAPE(N) \
-  case N: epb[0] = Staged<N>::EPB; break;
+  case N: epb[0] = Staged<N, false>::EPB; epb[1] = Staged<N, true>::EPB; break;
   DGK_F64_SHAPE(2) DGK_F64_SHAPE(3) DGK_F64_SHAPE(4) DGK_F64_SHAPE(5) DGK_F64_SHAPE(6)
   DGK_F64_SHAPE(7) DGK_F64_SHAPE(8) DGK_F64_SHAPE(9) DGK_F64_SHAPE(10) DGK_F64_SHAPE(11)
   DGK_F64_SHAPE(12) DGK_F64_SHAPE(13) DGK_F64_SHAPE(14) DGK_F64_SHAPE(15) DGK_F64_SHAPE(16)
 #undef DGK_F64_SHAPE
   default: return 0;
   }
-  return 1 + tile_f64_shapes(Nq, epb + 1, n - 1);
+  return kValuShapes + tile_f64_shapes(Nq, epb + kValuShapes, n - kValuShapes);
 }
 
 int stage_f64_select(int Nq, int cand) {

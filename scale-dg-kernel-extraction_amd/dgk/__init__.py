@@ -150,8 +150,8 @@ def tile_shapes(evaltype, p: int) -> list:
     """Elements per workgroup of each tile-shape candidate of a stage kernel."""
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
-    e, n = np.zeros(5, np.int32), C.c_int(0)
-    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 5, C.byref(n)))
+    e, n = np.zeros(6, np.int32), C.c_int(0)
+    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 6, C.byref(n)))
     return e[:n.value].tolist()
 
 
@@ -160,12 +160,12 @@ def kernel_name(evaltype, p: int, cand: int) -> str:
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
     nq = p + 1
-    valu = evaltype == EVAL_HIP and nq <= 16  # the FP64 VALU kernel is candidate 0
-    if valu and cand == 0:
-        return f"stage_f64<{nq}>"
+    nvalu = 2 if evaltype == EVAL_HIP and nq <= 16 else 0  # FP64 VALU kernels first
+    if cand < nvalu:
+        return f"stage_f64<{nq}, {'true' if cand == 1 else 'false'}>"
     shapes = tile_shapes(evaltype, p)
-    mfma = shapes[1:] if valu else shapes
-    mc = cand - 1 if valu else cand
+    mfma = shapes[nvalu:]
+    mc = cand - nvalu
     seq = nq > 16 or (mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or mfma[mc] == 1))
     prec = "PrecF64" if evaltype == EVAL_HIP else "PrecMixed"
     return f"stage_mfma<{prec}, {nq}, {mfma[mc]}, {'true' if seq else 'false'}>"
@@ -386,8 +386,8 @@ class Solver:
         """Time every tile shape of this solver's form and order on its mesh and
         keep the fastest for this solver."""
         best, n = C.c_int(0), C.c_int(0)
-        us = np.zeros(5)
-        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 5, C.byref(n)))
+        us = np.zeros(6)
+        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 6, C.byref(n)))
         return dict(best=best.value, us_per_launch=us[:n.value].tolist())
 
 

@@ -148,12 +148,12 @@ def test_tile_shapes_host_side():
     assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16, 4]
     # p >= 8: one element, two elements, one element direction by direction
     assert dgk.tile_shapes("HIP_MP", 15) == [1, 2, 1]
-    # FP64: the VALU kernel, then the FP64 MFMA kernel's shapes (two elements
-    # of p = 15 would need more than 160 KiB of LDS)
-    assert dgk.tile_shapes("HIP", 7) == [1, 1, 2, 4, 1]
-    assert dgk.tile_shapes("HIP", 15) == [1, 1, 1]
+    # FP64: the two VALU kernels, then the FP64 MFMA kernel's shapes (two
+    # elements of p = 15 would need more than 160 KiB of LDS)
+    assert dgk.tile_shapes("HIP", 7) == [1, 1, 1, 2, 4, 1]
+    assert dgk.tile_shapes("HIP", 15) == [1, 1, 1, 1]
     with pytest.raises(dgk.DgkError) as ei:
-        dgk.tile_select("HIP", 15, 3)
+        dgk.tile_select("HIP", 15, 4)
     assert ei.value.code == dgk.ERR_ARG
     with pytest.raises(dgk.DgkError) as ei:
         dgk.tile_select("HIP_MP", 15, 3)
@@ -165,13 +165,15 @@ def test_tile_shapes_host_side():
     assert dgk.tile_selected("HIP_MP", 7) == 2
     dgk.tile_select("HIP_MP", 7, -1)
     assert dgk.tile_selected("HIP_MP", 7) == 0
-    # FP64 defaults: the VALU kernel at p <= 7 and 10, 11, the
-    # direction-by-direction MFMA kernel (the last candidate) else
-    assert [dgk.tile_selected("HIP", p) for p in range(1, 16)] == [0] * 7 + [3, 3, 0, 0, 3, 3, 2, 2]
+    # FP64 defaults: the register VALU kernel at p = 3, 6, 7, 9..13, the
+    # direction-by-direction MFMA kernel (the last candidate) at p = 8, 14, 15,
+    # the staged VALU kernel else
+    assert [dgk.tile_selected("HIP", p) for p in range(1, 16)] == (
+        [0, 0, 1, 0, 0, 1, 1, 4, 1, 1, 1, 1, 1, 3, 3])
     dgk.tile_select("HIP", 13, 0)
     assert dgk.tile_selected("HIP", 13) == 0
     dgk.tile_select("HIP", 13, -1)
-    assert dgk.tile_selected("HIP", 13) == 3
+    assert dgk.tile_selected("HIP", 13) == 1
 
 
 def test_shapes_above_the_operator_files():
@@ -181,7 +183,8 @@ def test_shapes_above_the_operator_files():
         assert dgk.tile_shapes("HIP_MP", p) == [1]
         assert dgk.tile_selected("HIP", p) == 0
     assert dgk.kernel_name("HIP", 17, 0) == "stage_mfma<PrecF64, 18, 1, true>"
-    assert dgk.kernel_name("HIP", 7, 0) == "stage_f64<8>"
-    assert dgk.kernel_name("HIP", 7, 4) == "stage_mfma<PrecF64, 8, 1, true>"
+    assert dgk.kernel_name("HIP", 7, 0) == "stage_f64<8, false>"
+    assert dgk.kernel_name("HIP", 7, 1) == "stage_f64<8, true>"
+    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true>"
     assert dgk.kernel_name("HIP_MP", 15, 2) == "stage_mfma<PrecMixed, 16, 1, true>"
     assert dgk.element_sizes(19) == dict(Nq=20, Np=8000, Nfp=400, NfpTot=2400)
