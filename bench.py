@@ -87,6 +87,36 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 5}), flush=True)
 
 
+def copy_rate(nbytes=1 << 30, reps=10):
+    """Device-to-device hipMemcpyAsync of `nbytes`, timed with HIP events on the
+    null stream: (read + write bytes) / time in GB/s, the practical streaming
+    rate of this GPU beside the 8 TB/s datasheet peak."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+        assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+        assert hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None) == 0  # warm-up
+        assert hip.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            assert hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None) == 0
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` on this workload, measured by the
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (None when
@@ -258,24 +288,38 @@ def c3_leg(dgk, inputs):
 
 
 def cpu_baseline(wl, steps):
+    """The reference (oracle/_ref, GENERAL) on all host threads over `steps`
+    SSP-RK3 steps of the workload, plus two short companions: the reference on
+    one core, and the C restatement (oracle.c) on all threads, which shows that
+    the restatement is a faithful stand-in where the reference is not built."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     dof = wl.Ne * wl.Np
+    short = max(1, steps // 10)
     sample = (f"{wl.NeX}^3 elements p={wl.p} ({dof} DOF), {steps} SSP-RK3 steps of the "
-              f"same inputs (the GPU line times {wl.nstep}-step runs)")
+              f"same inputs (the GPU line times {wl.nstep}-step runs); companions run "
+              f"{short} steps")
+    m = oracle.Mesh(wl.NeX, wl.NeY, wl.NeZ, wl.p)
+    _, _, port_secs = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, short)
+    port = {"value": dof * short / port_secs, "cores": oracle.num_threads(), "steps": short}
     if oracle.reference_available():
         _, _, secs = oracle.run_reference("time", wl.NeX, wl.NeY, wl.NeZ, wl.p, "GENERAL",
                                           steps, wl.dt, wl.q, wl.u, wl.v, wl.w,
                                           threads=threads)
-        kind = "reference"
-    else:
-        m = oracle.Mesh(wl.NeX, wl.NeY, wl.NeZ, wl.p)
-        _, _, secs = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, steps)
-        kind = "port"
-    return {"value": dof * steps / secs, "unit": "DOF-updates/s", "cores": threads,
-            "kind": kind, "sample": sample, "seconds": secs}
+        _, _, secs1 = oracle.run_reference("time", wl.NeX, wl.NeY, wl.NeZ, wl.p, "GENERAL",
+                                           short, wl.dt, wl.q, wl.u, wl.v, wl.w, threads=1)
+        return {"value": dof * steps / secs, "unit": "DOF-updates/s", "cores": threads,
+                "kind": "reference", "sample": sample, "seconds": secs,
+                "one_core": {"value": dof * short / secs1, "steps": short},
+                "port_same_inputs": port}
+    port_secs_full = port_secs
+    if steps != short:
+        _, _, port_secs_full = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, steps)
+    return {"value": dof * steps / port_secs_full, "unit": "DOF-updates/s",
+            "cores": oracle.num_threads(), "kind": "port", "sample": sample,
+            "seconds": port_secs_full}
 
 
 def main():
@@ -321,6 +365,12 @@ def main():
         "cpu_baseline": None,
         "extra": {"q_minmax_after": r["minmax"], "device_timers": r["timers"]},
     }
+    # the practical streaming rate beside the datasheet peak, and the measured
+    # traffic's rate, for reading `frac`
+    roof = line["roofline"]
+    roof["copy_GBs"] = copy_rate()
+    if roof["traffic"]:
+        roof["traffic_GBs"] = roof["traffic"] / r["launch_s"] / 1e9
     if not args.no_extra:
         other = "HIP_MP" if args.form == "HIP" else "HIP"
         _, r2 = time_solver(dgk, inputs, other, args.steps, args.warmup)

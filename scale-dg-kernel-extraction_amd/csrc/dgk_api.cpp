@@ -1046,7 +1046,8 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
       if (p) p[g] = b->group_p[g];
       if (best) best[g] = arg;
       if (us_best) us_best[g] = t[arg];
-      if (us_default) us_default[g] = t[0];
+      const int def = tile_selected(s->evaltype, s->Nq);
+      if (us_default) us_default[g] = t[def >= 0 && def < nc ? def : 0];
     }
   }
   if (n) *n = (int)b->groups.size();
