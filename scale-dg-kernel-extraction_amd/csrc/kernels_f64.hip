@@ -103,6 +103,15 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   const int tid = threadIdx.x;
   const int64_t slot0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB;
 
+  // D and the 1D Lift into registers first; they are written to LDS after
+  // every other load is in flight, so no wait for them holds up the rest
+  constexpr int ND = (NQ2 + C::THREADS - 1) / C::THREADS;
+  double dreg[ND], lreg = 0.0;
+#pragma unroll
+  for (int r = 0; r < ND; ++r)
+    dreg[r] = tid + r * C::THREADS < NQ2 ? D_[tid + r * C::THREADS] : 0.0;
+  if (tid < 2 * NQ) lreg = ell_[tid];
+
   // ---- 1. async HBM -> LDS copy of q, u, v, w -------------------------------
   {
     const int wave = tid >> 6, lane = tid & 63;
@@ -131,9 +140,6 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
       }
     }
   }
-  for (int t = tid; t < NQ2; t += C::THREADS) sD[t] = D_[t];
-  for (int t = tid; t < 2 * NQ; t += C::THREADS) sL[t] = ell_[t];
-
   const bool lane_ok = tid < EPB * NQ2;
   const int el = lane_ok ? tid / NQ2 : 0;
   const int ij = lane_ok ? tid - el * NQ2 : 0;
@@ -165,11 +171,13 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   double qP[6], uP[6], vP[6], wP[6], q0v[NQ];
   const bool axal = axis_aligned_ != 0;
   if (active && !flux_in_) {
+    // all six neighbour indices first (one wait), then the gathers
+    int64_t pidx6[6];
+    nbr_indices<NQ>(vmapP_, nbr_, e, i, j, ij, pidx6);
     if (axal) {
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
-        const int64_t pidx = vmapP_ ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
-                                     : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+        const int64_t pidx = pidx6[f];
         const double *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
         qP[f] = q_[pidx];
         uP[f] = vel[pidx];
@@ -177,8 +185,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
     } else {
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
-        const int64_t pidx = vmapP_ ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
-                                     : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+        const int64_t pidx = pidx6[f];
         qP[f] = q_[pidx];
         uP[f] = u_[pidx];
         vP[f] = v_[pidx];
@@ -203,6 +210,10 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q0v[k] = __builtin_nontemporal_load(q0_ + base + (int64_t)k * NQ2);
   }
+#pragma unroll
+  for (int r = 0; r < ND; ++r)
+    if (tid + r * C::THREADS < NQ2) sD[tid + r * C::THREADS] = dreg[r];
+  if (tid < 2 * NQ) sL[tid] = lreg;
   __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
   __syncthreads();
 
