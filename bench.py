@@ -288,38 +288,45 @@ def c3_leg(dgk, inputs):
 
 
 def cpu_baseline(wl, steps):
-    """The reference (oracle/_ref, GENERAL) on all host threads over `steps`
-    SSP-RK3 steps of the workload, plus two short companions: the reference on
-    one core, and the C restatement (oracle.c) on all threads, which shows that
-    the restatement is a faithful stand-in where the reference is not built."""
+    """The reference (oracle/_ref, GENERAL) on all host threads: the median of
+    three runs of steps/3 SSP-RK3 steps each of the workload.  Two short
+    companions: the reference on one core, and the C restatement (oracle.c)
+    on all threads, which shows that the restatement is a faithful stand-in
+    where the reference is not built.  Threads are left unbound: binding with
+    OMP_PLACES=cores serialised the reference's threads in a container whose
+    CPU set is a slice of the host."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     dof = wl.Ne * wl.Np
+    per_run = max(1, steps // 3)
     short = max(1, steps // 10)
-    sample = (f"{wl.NeX}^3 elements p={wl.p} ({dof} DOF), {steps} SSP-RK3 steps of the "
-              f"same inputs (the GPU line times {wl.nstep}-step runs); companions run "
-              f"{short} steps")
+    sample = (f"{wl.NeX}^3 elements p={wl.p} ({dof} DOF), median of 3 runs of {per_run} "
+              f"SSP-RK3 steps of the same inputs (the GPU line times {wl.nstep}-step runs); "
+              f"companions run {short} steps")
     m = oracle.Mesh(wl.NeX, wl.NeY, wl.NeZ, wl.p)
     _, _, port_secs = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, short)
     port = {"value": dof * short / port_secs, "cores": oracle.num_threads(), "steps": short}
     if oracle.reference_available():
-        _, _, secs = oracle.run_reference("time", wl.NeX, wl.NeY, wl.NeZ, wl.p, "GENERAL",
-                                          steps, wl.dt, wl.q, wl.u, wl.v, wl.w,
-                                          threads=threads)
+        runs = []
+        for _ in range(3):
+            _, _, secs = oracle.run_reference("time", wl.NeX, wl.NeY, wl.NeZ, wl.p, "GENERAL",
+                                              per_run, wl.dt, wl.q, wl.u, wl.v, wl.w,
+                                              threads=threads)
+            runs.append(dof * per_run / secs)
         _, _, secs1 = oracle.run_reference("time", wl.NeX, wl.NeY, wl.NeZ, wl.p, "GENERAL",
                                            short, wl.dt, wl.q, wl.u, wl.v, wl.w, threads=1)
-        return {"value": dof * steps / secs, "unit": "DOF-updates/s", "cores": threads,
-                "kind": "reference", "sample": sample, "seconds": secs,
+        return {"value": float(np.median(runs)), "unit": "DOF-updates/s", "cores": threads,
+                "kind": "reference", "sample": sample, "runs": runs,
                 "one_core": {"value": dof * short / secs1, "steps": short},
                 "port_same_inputs": port}
-    port_secs_full = port_secs
-    if steps != short:
-        _, _, port_secs_full = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, steps)
-    return {"value": dof * steps / port_secs_full, "unit": "DOF-updates/s",
-            "cores": oracle.num_threads(), "kind": "port", "sample": sample,
-            "seconds": port_secs_full}
+    runs = []
+    for _ in range(3):
+        _, _, secs = m.run(wl.q, wl.u, wl.v, wl.w, wl.dt, per_run)
+        runs.append(dof * per_run / secs)
+    return {"value": float(np.median(runs)), "unit": "DOF-updates/s",
+            "cores": oracle.num_threads(), "kind": "port", "sample": sample, "runs": runs}
 
 
 def main():
