@@ -109,7 +109,7 @@ struct MP {
 // whole elements, then one wavefront direction by direction; above, one
 // element, two elements, and one element direction by direction.  Kinds
 // whose LDS exceeds a CU's 160 KiB are dropped; the remaining ones are the
-// candidates 0, 1, ... (0 is the default).
+// candidates 0, 1, ... (per-order defaults in kMpDefault below).
 constexpr int kKinds = 4;
 template <int NQ>
 constexpr int kind_epb(int k) {
@@ -451,8 +451,13 @@ struct Cand {
   static constexpr bool SEQ = kind_seq<NQ>(K < 0 ? 0 : K);
 };
 
-// Tile-shape candidate selected per Nq for the mixed form.
-int g_mp_shape[kMaxNq + 1] = {0};
+// Tile-shape candidate selected per Nq for the mixed form.  Defaults from
+// per-order timings on MI355X at about 1.7e7 DOF (whole-step runs, every
+// candidate): the one-wavefront shape, except the direction-by-direction
+// shape at p = 2 and p = 14 and the two-wavefront shape at p = 4.
+constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 0, 3, 0, 1, 0, 0, 0, 0, 0,
+                                        0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
+int g_mp_shape[kMaxNq + 1] = {0, 0, 0, 3, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
 
 template <class P, int NQ>
 int prepare_nq_all() {
@@ -545,7 +550,7 @@ int stage_mp_select(int Nq, int cand) {
   int e[kKinds];
   const int nc = stage_mp_shapes(Nq, e, kKinds);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
-  if (cand == -1) cand = 0;  // the default
+  if (cand == -1) cand = kMpDefault[Nq];
   if (cand < 0 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
   g_mp_shape[Nq] = cand;

@@ -165,6 +165,14 @@ def test_tile_shapes_host_side():
     assert dgk.tile_selected("HIP_MP", 7) == 2
     dgk.tile_select("HIP_MP", 7, -1)
     assert dgk.tile_selected("HIP_MP", 7) == 0
+    # mixed defaults: one wavefront, except direction by direction at p = 2
+    # and 14 and two wavefronts at p = 4
+    assert [dgk.tile_selected("HIP_MP", p) for p in range(1, 16)] == (
+        [0, 3, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
+    dgk.tile_select("HIP_MP", 4, 0)
+    assert dgk.tile_selected("HIP_MP", 4) == 0
+    dgk.tile_select("HIP_MP", 4, -1)
+    assert dgk.tile_selected("HIP_MP", 4) == 1
     # FP64 defaults: the register VALU kernel at p = 3, 6, 7, 9..13, the
     # direction-by-direction MFMA kernel (the last candidate) at p = 8, 14, 15,
     # the staged VALU kernel else
