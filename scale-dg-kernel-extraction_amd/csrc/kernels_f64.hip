@@ -634,13 +634,14 @@ unsigned grid1d(int64_t n, int threads) {
 // launches that pass a flux (cal_dqdt) or need the stored Lift take the VALU
 // kernels (stage_f64 / stage_f64_lift).
 constexpr int kValuShapes = 2;
-// Default per Nq, from per-order timings on MI355X at 2e6..1.7e7 DOF: the
-// register VALU kernel at p = 3, 6, 7, 9..13, the direction-by-direction MFMA
-// kernel (the last candidate) at p = 8, 14, 15, the staged VALU kernel else.
-constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 4, 1, 1, 1, 1, 1, 3, 3,
+// Default per Nq, from whole-step timings of every candidate on MI355X at
+// about 1.7e7 DOF: the register VALU kernel at p = 3 and 6..13, the
+// direction-by-direction MFMA kernel (the last candidate) at p = 14, 15, the
+// staged VALU kernel else.
+constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
                                          0, 0, 0, 0};
 // Selected FP64 tile shape per Nq (see stage_f64_shapes).
-int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 4, 1, 1, 1, 1, 1, 3, 3, 0, 0, 0, 0};
+int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 0, 0, 0, 0};
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
