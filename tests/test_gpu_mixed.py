@@ -149,6 +149,11 @@ def test_autotune_solver_and_batch():
         assert sorted(x["p"] for x in rec) == sorted({it["p"] for it in ens})
         for x in rec:
             assert x["epb"] in dgk.tile_shapes("HIP_MP", x["p"]) and x["us_per_launch"] > 0
+            # the default's time is the default candidate's own timing
+            if x["candidate"] == dgk.tile_selected("HIP_MP", x["p"]):
+                assert x["us_default"] == x["us_per_launch"]
+            else:
+                assert x["us_per_launch"] <= 0.98 * x["us_default"]
         b.step(1)  # re-captured with the tuned shapes
         for k in range(len(ens)):
             assert np.array_equal(b.get_field(k), ref.get_field(k)), k
