@@ -71,27 +71,26 @@ __host__ __device__ inline int nbr_face_node(int f, int a, int b, int Nq) {
 }
 
 #if defined(__HIPCC__)
-// Flat indices of the neighbour nodes that face node (i,j) of element e meets
-// on its six faces: from the resolved per-face-node table when there is one,
-// else from the element neighbour table.  All six table entries are loaded
-// before any is used, so their latencies overlap.
+// Neighbour lookups of face node (i,j) of element e on its six faces: raw
+// entries of the resolved per-face-node table when there is one, else of the
+// element neighbour table.  All six entries are loaded before any is used, so
+// their latencies overlap; nbr_node turns an entry into a flat node index.
 template <int NQ>
-__device__ __forceinline__ void nbr_indices(const int32_t *__restrict__ vmapP,
-                                            const int32_t *__restrict__ nbr, int64_t e, int i,
-                                            int j, int ij, int64_t (&pidx)[6]) {
-  constexpr int NQ2 = NQ * NQ, NP = NQ2 * NQ, NFT = 6 * NQ2;
-  int32_t t[6];
+__device__ __forceinline__ void nbr_entries(const int32_t *__restrict__ vmapP,
+                                            const int32_t *__restrict__ nbr, int64_t e, int ij,
+                                            int32_t (&t)[6]) {
+  constexpr int NQ2 = NQ * NQ, NFT = 6 * NQ2;
   if (vmapP) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) t[f] = vmapP[e * NFT + f * NQ2 + ij];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) pidx[f] = (int64_t)t[f];
   } else {
 #pragma unroll
     for (int f = 0; f < 6; ++f) t[f] = nbr[e * 6 + f];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) pidx[f] = (int64_t)t[f] * NP + nbr_face_node(f, i, j, NQ);
   }
+}
+template <int NQ>
+__device__ __forceinline__ int64_t nbr_node(bool resolved, int32_t t, int f, int i, int j) {
+  return resolved ? (int64_t)t : (int64_t)t * (NQ * NQ * NQ) + nbr_face_node(f, i, j, NQ);
 }
 #endif
 

@@ -78,10 +78,15 @@ struct Staged {
   // (lane (i,j) writes where lane (j,i) read in the same instruction slot);
   // the x-contraction then reads consecutive addresses.
   static constexpr bool FX_T = WAVES == 1;
+  // Waves per SIMD the register allocation must leave room for: 3 for the
+  // register form at p = 11, which otherwise lands one VGPR above the
+  // 168 that 3 waves allow (and drops to 2).
+  static constexpr int MIN_WAVES_EU = (VREG && NQ == 12) ? 3 : 1;
 };
 
 template <int NQ, bool VREG>
-__global__ void __launch_bounds__((Staged<NQ, VREG>::THREADS))
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, VREG>::THREADS),
+                               amdgpu_waves_per_eu(Staged<NQ, VREG>::MIN_WAVES_EU)))
 stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
           const double *__restrict__ v_, const double *__restrict__ w_, double *out_,
           const double *__restrict__ flux_in_, const int32_t *__restrict__ nbr_,
@@ -102,15 +107,6 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 
   const int tid = threadIdx.x;
   const int64_t slot0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB;
-
-  // D and the 1D Lift into registers first; they are written to LDS after
-  // every other load is in flight, so no wait for them holds up the rest
-  constexpr int ND = (NQ2 + C::THREADS - 1) / C::THREADS;
-  double dreg[ND], lreg = 0.0;
-#pragma unroll
-  for (int r = 0; r < ND; ++r)
-    dreg[r] = tid + r * C::THREADS < NQ2 ? D_[tid + r * C::THREADS] : 0.0;
-  if (tid < 2 * NQ) lreg = ell_[tid];
 
   // ---- 1. async HBM -> LDS copy of q, u, v, w -------------------------------
   {
@@ -171,13 +167,24 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   double qP[6], uP[6], vP[6], wP[6], q0v[NQ];
   const bool axal = axis_aligned_ != 0;
   if (active && !flux_in_) {
-    // all six neighbour indices first (one wait), then the gathers
-    int64_t pidx6[6];
-    nbr_indices<NQ>(vmapP_, nbr_, e, i, j, ij, pidx6);
+    // With several elements per workgroup the table entries are per-lane
+    // loads: all six go out before the gathers.  With one, they are scalar
+    // loads, read where used (loading them ahead costs a register, which at
+    // p = 11 drops the kernel from 3 to 2 waves per SIMD).
+    const bool resolved = vmapP_ != nullptr;
+    int32_t nb6[6];
+    if constexpr (EPB > 1) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
+    auto nbr_at = [&](int f) -> int64_t {
+      if constexpr (EPB > 1)
+        return nbr_node<NQ>(resolved, nb6[f], f, i, j);
+      else
+        return resolved ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
+                        : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+    };
     if (axal) {
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
-        const int64_t pidx = pidx6[f];
+        const int64_t pidx = nbr_at(f);
         const double *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
         qP[f] = q_[pidx];
         uP[f] = vel[pidx];
@@ -185,7 +192,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
     } else {
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
-        const int64_t pidx = pidx6[f];
+        const int64_t pidx = nbr_at(f);
         qP[f] = q_[pidx];
         uP[f] = u_[pidx];
         vP[f] = v_[pidx];
@@ -210,10 +217,10 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q0v[k] = __builtin_nontemporal_load(q0_ + base + (int64_t)k * NQ2);
   }
-#pragma unroll
-  for (int r = 0; r < ND; ++r)
-    if (tid + r * C::THREADS < NQ2) sD[tid + r * C::THREADS] = dreg[r];
-  if (tid < 2 * NQ) sL[tid] = lreg;
+  // D and the 1D Lift go to LDS last: the wait for them comes after every
+  // other load is in flight
+  for (int t = tid; t < NQ2; t += C::THREADS) sD[t] = D_[t];
+  for (int t = tid; t < 2 * NQ; t += C::THREADS) sL[t] = ell_[t];
   __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
   __syncthreads();
 

@@ -191,7 +191,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   using C = MP<P, NQ, EPB, SEQ>;
   using vel_t = typename P::vel_t;
   using op_t = typename P::op_t;
-  constexpr int NQ2 = C::NQ2, NP = C::NP, KPAD = C::KPAD, KSTEPS = C::KSTEPS;
+  constexpr int NQ2 = C::NQ2, NP = C::NP, NFT = 6 * NQ2, KPAD = C::KPAD, KSTEPS = C::KSTEPS;
   constexpr int AREA = C::AREA;
   extern __shared__ double smem_mp[];
   op_t *const As = (op_t *)smem_mp;
@@ -228,7 +228,16 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   vel_t uc[SEQ ? 1 : NQ], vc[SEQ ? 1 : NQ], wc[NQ];  // SEQ: w only at k = 0, NQ-1
   double qM[4], qP[6];
   vel_t vM[4], vP[6];  // velocity component along the face axis
-  int64_t pidx6[6];
+  // Neighbour table entries: with several elements per workgroup they are
+  // per-lane loads, all six issued before the gathers; with one they are
+  // scalar loads read where used (fewer live registers at high order).
+  int32_t nb6[6] = {};
+  const bool resolved = vmapP_ != nullptr;
+  auto nbr_at = [&](int f) -> int64_t {
+    if constexpr (EPB > 1) return nbr_node<NQ>(resolved, nb6[f], f, i, j);
+    return resolved ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
+                    : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+  };
   if (active) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
@@ -249,12 +258,12 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       qM[f] = q_[m];
       vM[f] = (kFaceAxis[f] == 0 ? u_ : v_)[m];
     }
-    // all six neighbour indices first (one wait), then the gathers
-    nbr_indices<NQ>(vmapP_, nbr_, e, i, j, ij, pidx6);
+    if constexpr (EPB > 1) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-      qP[f] = q_[pidx6[f]];
-      vP[f] = (kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_)[pidx6[f]];
+      const int64_t pidx = nbr_at(f);
+      qP[f] = q_[pidx];
+      vP[f] = (kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_)[pidx];
     }
   }
 
@@ -274,7 +283,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       } else {
         // general normals: all three components on both sides
         const int64_t m = ebase + own_face_node(f, i, j, NQ);
-        const int64_t pidx = pidx6[f];
+        const int64_t pidx = nbr_at(f);
         const double n1 = g[kGeoNormal + 3 * f], n2 = g[kGeoNormal + 3 * f + 1],
                      n3 = g[kGeoNormal + 3 * f + 2];
         VelM = (double)u_[m] * n1 + (double)v_[m] * n2 + (double)w_[m] * n3;
