@@ -108,6 +108,40 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   const int tid = threadIdx.x;
   const int64_t slot0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB;
 
+  const bool lane_ok = tid < EPB * NQ2;
+  const int el = lane_ok ? tid / NQ2 : 0;
+  const int ij = lane_ok ? tid - el * NQ2 : 0;
+  const int i = ij % NQ, j = ij / NQ;
+  const int64_t slot = slot0 + el;
+  const bool active = lane_ok && slot < Ne_;
+  int64_t e;
+  const double *g;
+  if (EPB == 1) {
+    // one element per workgroup: its index, neighbour table entries and
+    // geometry record are wave-uniform and read through the scalar cache
+    e = slot < Ne_ ? (order_ ? (int64_t)order_[slot] : slot) : 0;
+    e = uniform(e);
+    g = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+  } else {
+    e = active ? (order_ ? (int64_t)order_[slot] : slot) : 0;
+    g = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+  }
+  const int64_t base = e * NP + ij;
+  const int lb = el * NP;  // element block in the staged fields
+  const bool rk = mode_ == 1;
+  const bool use_q0 = rk && ca_ != 0.0;
+
+  // With several elements per workgroup the neighbour-table entries are
+  // per-lane loads: issued first, so waiting for them before the trace
+  // gathers does not also wait for the async copies behind them.  With one
+  // element they are scalar loads, read where used (loading them ahead costs
+  // a register, which at p = 11 drops the kernel from 3 to 2 waves per SIMD).
+  const bool resolved = vmapP_ != nullptr;
+  int32_t nb6[6];
+  if constexpr (EPB > 1) {
+    if (active && !flux_in_) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
+  }
+
   // ---- 1. async HBM -> LDS copy of q, u, v, w -------------------------------
   {
     const int wave = tid >> 6, lane = tid & 63;
@@ -136,28 +170,6 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
       }
     }
   }
-  const bool lane_ok = tid < EPB * NQ2;
-  const int el = lane_ok ? tid / NQ2 : 0;
-  const int ij = lane_ok ? tid - el * NQ2 : 0;
-  const int i = ij % NQ, j = ij / NQ;
-  const int64_t slot = slot0 + el;
-  const bool active = lane_ok && slot < Ne_;
-  int64_t e;
-  const double *g;
-  if (EPB == 1) {
-    // one element per workgroup: its index, neighbour table entries and
-    // geometry record are wave-uniform and read through the scalar cache
-    e = slot < Ne_ ? (order_ ? (int64_t)order_[slot] : slot) : 0;
-    e = uniform(e);
-    g = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
-  } else {
-    e = active ? (order_ ? (int64_t)order_[slot] : slot) : 0;
-    g = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
-  }
-  const int64_t base = e * NP + ij;
-  const int lb = el * NP;  // element block in the staged fields
-  const bool rk = mode_ == 1;
-  const bool use_q0 = rk && ca_ != 0.0;
 
   // neighbour traces of face node (i,j) on all six faces, and the RK base
   // column, in flight together with the async copy
@@ -167,13 +179,6 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   double qP[6], uP[6], vP[6], wP[6], q0v[NQ];
   const bool axal = axis_aligned_ != 0;
   if (active && !flux_in_) {
-    // With several elements per workgroup the table entries are per-lane
-    // loads: all six go out before the gathers.  With one, they are scalar
-    // loads, read where used (loading them ahead costs a register, which at
-    // p = 11 drops the kernel from 3 to 2 waves per SIMD).
-    const bool resolved = vmapP_ != nullptr;
-    int32_t nb6[6];
-    if constexpr (EPB > 1) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
     auto nbr_at = [&](int f) -> int64_t {
       if constexpr (EPB > 1)
         return nbr_node<NQ>(resolved, nb6[f], f, i, j);

@@ -224,20 +224,25 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   const vel_t *const velp[3] = {u_, v_, w_};
 
   // ---- 1. loads straight into registers ------------------------------------
-  double qc[NQ];
-  vel_t uc[SEQ ? 1 : NQ], vc[SEQ ? 1 : NQ], wc[NQ];  // SEQ: w only at k = 0, NQ-1
-  double qM[4], qP[6];
-  vel_t vM[4], vP[6];  // velocity component along the face axis
-  // Neighbour table entries: with several elements per workgroup they are
-  // per-lane loads, all six issued before the gathers; with one they are
-  // scalar loads read where used (fewer live registers at high order).
+  // Neighbour-table entries: with several elements per workgroup they are
+  // per-lane loads, issued first so that waiting for them before the trace
+  // gathers does not also wait for the column loads behind them; with one
+  // they are scalar loads read where used (fewer live registers at high
+  // order).
   int32_t nb6[6] = {};
   const bool resolved = vmapP_ != nullptr;
+  if constexpr (EPB > 1) {
+    if (active) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
+  }
   auto nbr_at = [&](int f) -> int64_t {
     if constexpr (EPB > 1) return nbr_node<NQ>(resolved, nb6[f], f, i, j);
     return resolved ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
                     : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
   };
+  double qc[NQ];
+  vel_t uc[SEQ ? 1 : NQ], vc[SEQ ? 1 : NQ], wc[NQ];  // SEQ: w only at k = 0, NQ-1
+  double qM[4], qP[6];
+  vel_t vM[4], vP[6];  // velocity component along the face axis
   if (active) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
@@ -258,7 +263,6 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       qM[f] = q_[m];
       vM[f] = (kFaceAxis[f] == 0 ? u_ : v_)[m];
     }
-    if constexpr (EPB > 1) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       const int64_t pidx = nbr_at(f);
