@@ -14,7 +14,8 @@ The JSON line also carries
                 from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/;
   cpu_baseline  the reference Fortran GENERAL path (oracle/_ref, OpenMP on the
                 host cores) on a bounded number of steps of the same mesh;
-  extra         the mixed-precision form on the same mesh; the batched
+  extra         the mixed-precision form on the same mesh; the same run with
+                the host upload and download inside the clock; the batched
                 ensemble (512 seeded instances, Ne/axis in {4,6,8,12,16},
                 p in {3,5,7,11}, 50 steps, graph-replayed) in FP64 and mixed
                 precision against the plain FP64 path run instance by
@@ -155,6 +156,30 @@ def time_solver(dgk, inputs, form, steps, warmup, autotune=True):
     stage_s, stage_n = tm["Stage (fused)"]
     return wl, dict(value=dof * steps / secs, seconds=secs, launch_s=stage_s / max(stage_n, 1),
                     launches=stage_n, timers=tm, minmax=[qmin, qmax], dof=dof, tune=tune)
+
+
+def pcie_leg(dgk, inputs, form, steps):
+    """The same run with the host boundary inside the clock: q, u, v, w
+    uploaded from host memory (dgk_set_field), `steps` steps, q downloaded
+    (dgk_get_field).  Reported beside `value`, never as it."""
+    wl = inputs.c2_gaussian()
+    dof = wl.Ne * wl.Np
+    with dgk.Solver.structured(form, wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)  # first touch of the device buffers
+        s.step(wl.dt, 2)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        s.synchronize()
+        t1 = time.perf_counter()
+        s.step(wl.dt, steps)
+        s.synchronize()
+        t2 = time.perf_counter()
+        s.get_field(dgk.FIELD_Q)
+        t3 = time.perf_counter()
+    return {"form": form, "steps": steps, "DOF-updates/s": dof * steps / (t3 - t0),
+            "upload_s": t1 - t0, "steps_s": t2 - t1, "download_s": t3 - t2,
+            "host_bytes": 5 * dof * 8}
 
 
 def batch_leg(dgk, inputs):
@@ -385,6 +410,7 @@ def main():
             "DOF-updates/s": r2["value"], "avg_launch_us": 1e6 * r2["launch_s"],
             "achieved_GBs": BYTES_PER_DOF_UPDATE[other] / 3.0 * dof / r2["launch_s"] / 1e9,
             "ratio_vs_" + args.form: r2["value"] / r["value"]}
+        line["extra"]["pcie_inclusive"] = pcie_leg(dgk, inputs, args.form, args.steps)
         line["extra"]["batched_ensemble"] = batch_leg(dgk, inputs)
         line["extra"]["c3_p15"] = c3_leg(dgk, inputs)
         line["extra"]["per_order"] = per_order_leg(dgk)
