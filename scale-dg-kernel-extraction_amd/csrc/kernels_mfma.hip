@@ -100,7 +100,18 @@ struct MP {
   static constexpr int KPAD = 4 * KSTEPS;
   static constexpr int AREA = ROWS * KPAD;                  // operands per direction
   static constexpr int NAREA = SEQ ? 1 : 3;
-  static constexpr size_t LDS_BYTES = (size_t)NAREA * AREA * sizeof(typename P::op_t);
+  static constexpr size_t A_BYTES = (size_t)NAREA * AREA * sizeof(typename P::op_t);
+  // One-wavefront all-at-once shapes first pass the q, u, v columns through
+  // LDS, where each lane picks up its own x/y face traces (same area as A,
+  // used before it); the others load those traces from global memory (the
+  // exchange would cost them registers and a wave per SIMD)
+  static constexpr bool XCH = !SEQ && WAVES == 1;
+  static constexpr size_t XCH_BYTES =
+      XCH ? (size_t)LANES * NQ * (sizeof(double) + 2 * sizeof(typename P::vel_t)) : 0;
+  static constexpr size_t LDS_BYTES = A_BYTES > XCH_BYTES ? A_BYTES : XCH_BYTES;
+  // Register budget: the mixed exchange shapes keep 4 waves per SIMD (at most
+  // 128 registers; left free, p = 7 lands two above and drops to 3)
+  static constexpr int MIN_WAVES_EU = (sizeof(typename P::op_t) == 4 && XCH) ? 4 : 1;
   static constexpr int JOBS = NAREA * MTILES;               // (direction, tile) GEMMs per pass
   static constexpr int JPW = (JOBS + WAVES - 1) / WAVES;
 };
@@ -178,7 +189,8 @@ __device__ __forceinline__ void mfma_tile(typename P::op_t *tile,
 }
 
 template <class P, int NQ, int EPB, bool SEQ>
-__global__ void __launch_bounds__((MP<P, NQ, EPB, SEQ>::THREADS))
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, MP<P, NQ, EPB, SEQ>::THREADS),
+                               amdgpu_waves_per_eu(MP<P, NQ, EPB, SEQ>::MIN_WAVES_EU)))
 stage_mfma(const double *__restrict__ q_, const double *q0_,
            const typename P::vel_t *__restrict__ u_, const typename P::vel_t *__restrict__ v_,
            const typename P::vel_t *__restrict__ w_, double *out_,
@@ -257,11 +269,13 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       wc[0] = w_[base];
       wc[NQ - 1] = w_[base + (NQ - 1) * NQ2];
     }
+    if constexpr (!C::XCH) {
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {  // own traces of the x and y faces
-      const int64_t m = ebase + own_face_node(f, i, j, NQ);
-      qM[f] = q_[m];
-      vM[f] = (kFaceAxis[f] == 0 ? u_ : v_)[m];
+      for (int f = 0; f < 4; ++f) {  // own traces of the x and y faces
+        const int64_t m = ebase + own_face_node(f, i, j, NQ);
+        qM[f] = q_[m];
+        vM[f] = (kFaceAxis[f] == 0 ? u_ : v_)[m];
+      }
     }
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
@@ -269,6 +283,34 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       qP[f] = q_[pidx];
       vP[f] = (kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_)[pidx];
     }
+  }
+
+  // ---- 1b. own x/y face traces from the lanes that hold them (XCH) --------
+  // Face node (a,b) of face f lies in the column of lane (a,0), (NQ-1,a),
+  // (a,NQ-1) or (0,a) at k = b; every lane stores its q, u, v columns in LDS
+  // and reads its four traces back, with no global loads (they would be
+  // gathers of one cache line per two lanes on the x faces).
+  if constexpr (C::XCH) {
+    double *xq = (double *)smem_mp;
+    vel_t *xu = (vel_t *)(xq + C::LANES * NQ), *xv = xu + C::LANES * NQ;
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        xq[tid * NQ + k] = qc[k];
+        xu[tid * NQ + k] = uc[k];
+        xv[tid * NQ + k] = vc[k];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const int src[4] = {R + i, R + (NQ - 1) + i * NQ, R + i + (NQ - 1) * NQ, R + i * NQ};
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        qM[f] = xq[src[f] * NQ + j];
+        vM[f] = (kFaceAxis[f] == 0 ? xu : xv)[src[f] * NQ + j];
+      }
+    }
+    __syncthreads();  // the A operands below reuse the area
   }
 
   // ---- 2a. FP64 upwind fluxes -------------------------------------------------
