@@ -125,6 +125,18 @@ void fill_box_geo(double dx, double dy, double dz, double *rec);
 // elements (bricks x-fastest, elements x-fastest inside a brick).  Only the
 // assignment of elements to workgroups changes, never the arithmetic.
 std::vector<int32_t> brick_order(int NeX, int NeY, int NeZ, int bx, int by, int bz);
+// Element processing order.  The stage kernels deal consecutive slots of a
+// launch to one XCD (an eighth of the slots each).  For a mesh numbered
+// plane by plane along z with periodic +z neighbours e -> (e + P) mod Ne
+// (the reference's numbering, mod_mesh.f90:491) and a multiple of 8 planes,
+// the default order gives XCD x the planes x, x+8, x+16, ...: adjacent planes
+// then run at the same time on neighbouring XCDs, and the face traces they
+// share are fetched from HBM once while still in the memory-side cache.  Only
+// the assignment of elements to workgroups changes, never the arithmetic.
+// DGK_ORDER=identity keeps the element numbering.
+int64_t z_plane_size(const MeshTables &t);  // P, or 0 when not of that form
+std::vector<int32_t> zplane_interleaved_order(int64_t P, int64_t NeZ);
+void default_order(MeshTables &t);
 
 int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
                                 int64_t NhaloNode, const double *D1D,
