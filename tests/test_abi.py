@@ -166,13 +166,13 @@ def test_tile_shapes_host_side():
     dgk.tile_select("HIP_MP", 7, -1)
     assert dgk.tile_selected("HIP_MP", 7) == 0
     # mixed defaults: one wavefront, except direction by direction at p = 2
-    # and 14 and two wavefronts at p = 4
+    # and 14
     assert [dgk.tile_selected("HIP_MP", p) for p in range(1, 16)] == (
-        [0, 3, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
-    dgk.tile_select("HIP_MP", 4, 0)
-    assert dgk.tile_selected("HIP_MP", 4) == 0
-    dgk.tile_select("HIP_MP", 4, -1)
-    assert dgk.tile_selected("HIP_MP", 4) == 1
+        [0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
+    dgk.tile_select("HIP_MP", 2, 0)
+    assert dgk.tile_selected("HIP_MP", 2) == 0
+    dgk.tile_select("HIP_MP", 2, -1)
+    assert dgk.tile_selected("HIP_MP", 2) == 3
     # FP64 defaults: the register VALU kernel at p = 3 and 6..13, the
     # direction-by-direction MFMA kernel (the last candidate) at p = 14, 15,
     # the staged VALU kernel else
