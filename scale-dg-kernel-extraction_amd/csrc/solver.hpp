@@ -36,6 +36,8 @@ struct dgk_solver {
   float *D32 = nullptr, *ell32 = nullptr;
   double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
   int32_t *order = nullptr;  // element processing order
+  int32_t *order_rev = nullptr;  // the same order backwards (every other launch)
+  uint64_t launches = 0;         // stage launches so far (picks the direction)
   int axis_aligned = 0;
   int tile = -1;             // tile-shape candidate chosen for this mesh (-1: process default)
   double *x1D = nullptr;     // LGL nodes (device), for device-side initial conditions
