@@ -108,7 +108,14 @@ struct MP {
   static constexpr bool XCH = !SEQ && WAVES == 1;
   static constexpr size_t XCH_BYTES =
       XCH ? (size_t)LANES * NQ * (sizeof(double) + 2 * sizeof(typename P::vel_t)) : 0;
-  static constexpr size_t LDS_BYTES = A_BYTES > XCH_BYTES ? A_BYTES : XCH_BYTES;
+  // Multi-wavefront all-at-once shapes up to p = 9: the face lanes put the
+  // x/y face traces into a buffer after the A area, and the x/y fluxes are
+  // formed after the barrier that already precedes the GEMMs (one barrier
+  // more; above p = 9 it cost more than the global trace loads it saves)
+  static constexpr bool XD = !SEQ && WAVES > 1 && NQ <= 10;
+  static constexpr size_t XD_BYTES =
+      XD ? (size_t)EPB * 4 * NQ2 * (sizeof(double) + sizeof(typename P::vel_t)) : 0;
+  static constexpr size_t LDS_BYTES = (A_BYTES > XCH_BYTES ? A_BYTES : XCH_BYTES) + XD_BYTES;
   // Register budget: the mixed exchange shapes keep 4 waves per SIMD (at most
   // 128 registers; left free, p = 7 lands two above and drops to 3)
   static constexpr int MIN_WAVES_EU = (sizeof(typename P::op_t) == 4 && XCH) ? 4 : 1;
@@ -269,7 +276,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       wc[0] = w_[base];
       wc[NQ - 1] = w_[base + (NQ - 1) * NQ2];
     }
-    if constexpr (!C::XCH) {
+    if constexpr (!C::XCH && !C::XD) {
 #pragma unroll
       for (int f = 0; f < 4; ++f) {  // own traces of the x and y faces
         const int64_t m = ebase + own_face_node(f, i, j, NQ);
@@ -313,11 +320,23 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
     __syncthreads();  // the A operands below reuse the area
   }
 
+  double *xd_q = (double *)((char *)smem_mp + C::A_BYTES) + el * 4 * NQ2;
+  vel_t *xd_v = (vel_t *)((double *)((char *)smem_mp + C::A_BYTES) + EPB * 4 * NQ2) + el * 4 * NQ2;
+  if constexpr (C::XD) {
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        if (j == 0) { xd_q[0 * NQ2 + i + k * NQ] = qc[k]; xd_v[0 * NQ2 + i + k * NQ] = vc[k]; }
+        if (i == NQ - 1) { xd_q[1 * NQ2 + j + k * NQ] = qc[k]; xd_v[1 * NQ2 + j + k * NQ] = uc[k]; }
+        if (j == NQ - 1) { xd_q[2 * NQ2 + i + k * NQ] = qc[k]; xd_v[2 * NQ2 + i + k * NQ] = vc[k]; }
+        if (i == 0) { xd_q[3 * NQ2 + j + k * NQ] = qc[k]; xd_v[3 * NQ2 + j + k * NQ] = uc[k]; }
+      }
+    }
+  }
   // ---- 2a. FP64 upwind fluxes -------------------------------------------------
   op_t fl[6];
-  if (active) {
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
+  auto face_flux = [&](int f) {
+    {
       const double qm = f == 4 ? qc[0] : f == 5 ? qc[NQ - 1] : qM[f];
       double VelM, VelP;
       if (axal) {
@@ -338,6 +357,10 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       const double alpha = 0.5 * fabs(VelP + VelM);
       fl[f] = (op_t)(0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qm * VelM - alpha * (qP[f] - qm)));
     }
+  };
+  if (active) {
+#pragma unroll
+    for (int f = C::XD ? 4 : 0; f < 6; ++f) face_flux(f);
   }
 
   // B = [D^T ; l- ; l+ ; 0] rows of this lane (k permuted, see above)
@@ -378,8 +401,10 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
           a[k * pos_step[d]] = (op_t)(Ed * (qc[k] * (double)vel));
         }
         op_t *r = As + d * AREA + (R + ij) * KPAD;
-        r[NQ] = fl[face_m[d]];
-        r[NQ + 1] = fl[face_p[d]];
+        if (!C::XD || d == 2) {
+          r[NQ] = fl[face_m[d]];
+          r[NQ + 1] = fl[face_p[d]];
+        }
 #pragma unroll
         for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
       }
@@ -392,6 +417,23 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       for (int k = 0; k < NQ; ++k) q0v[k] = __builtin_nontemporal_load(q0_ + base + k * NQ2);
     }
     __syncthreads();
+    if constexpr (C::XD) {
+      if (active) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          qM[f] = xd_q[f * NQ2 + ij];
+          vM[f] = xd_v[f * NQ2 + ij];
+          face_flux(f);
+        }
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          op_t *r = As + d * AREA + (R + ij) * KPAD;
+          r[NQ] = fl[face_m[d]];
+          r[NQ + 1] = fl[face_p[d]];
+        }
+      }
+      __syncthreads();
+    }
 
     // ---- 3. MFMA GEMMs over (direction, 16-pencil tile) jobs ---------------
     // Rows of A beyond the block's active pencils hold stale LDS; each output
