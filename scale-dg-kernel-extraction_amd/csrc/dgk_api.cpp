@@ -632,8 +632,12 @@ struct dgk_batch {
   std::vector<Loc> loc;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // Two captured steps, replayed in turn: stage launches alternate their
+  // direction, so step 2k runs forwards-backwards-forwards and step 2k+1
+  // backwards-forwards-backwards, as an uncaptured sequence of steps would.
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int next = 0;  // the captured step to replay next
   bool use_graph = true;
   double step_seconds = 0.0;
   int64_t step_calls = 0;
@@ -672,18 +676,38 @@ int batch_enqueue_step(dgk_batch *b) {
   return DGK_OK;
 }
 
-int batch_capture(dgk_batch *b) {
-  DGK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = batch_enqueue_step(b);
-  hipGraph_t graph = nullptr;
-  const hipError_t e = hipStreamEndCapture(b->stream, &graph);
-  if (rc != DGK_OK) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
+void batch_release_graphs(dgk_batch *b) {
+  for (int k = 0; k < 2; ++k) {
+    if (b->exec[k]) (void)hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    b->exec[k] = nullptr;
+    b->graph[k] = nullptr;
   }
-  if (e != hipSuccess) return set_error(DGK_ERR_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));
-  b->graph = graph;
-  DGK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->next = 0;
+}
+
+int batch_capture(dgk_batch *b) {
+  for (int k = 0; k < 2; ++k) {
+    DGK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue_step(b);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc != DGK_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      batch_release_graphs(b);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      batch_release_graphs(b);
+      return set_error(DGK_ERR_DEVICE, std::string("stream capture: ") + hipGetErrorString(e));
+    }
+    b->graph[k] = graph;
+    if (hipGraphInstantiate(&b->exec[k], b->graph[k], nullptr, nullptr, 0) != hipSuccess) {
+      batch_release_graphs(b);
+      return set_error(DGK_ERR_DEVICE, "hipGraphInstantiate failed");
+    }
+  }
+  b->next = 0;
   return DGK_OK;
 }
 
@@ -707,8 +731,7 @@ int dgk_batch_destroy(dgk_batch *b) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
   }
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  batch_release_graphs(b);
   for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
   for (dgk_solver *s : b->groups) dgk_destroy(s);
@@ -882,14 +905,18 @@ int dgk_batch_step(dgk_batch *b, int nstep) {
   if (nstep == 0) return DGK_OK;
   // fields uploaded on the group streams are ordered before the step
   for (dgk_solver *s : b->groups) DGK_HIP(hipStreamSynchronize(s->stream));
-  if (b->use_graph && !b->exec) DGK_TRY(batch_capture(b));
+  if (b->use_graph && !b->exec[0]) DGK_TRY(batch_capture(b));
   hipEvent_t t0 = nullptr, t1 = nullptr;
   DGK_HIP(hipEventCreate(&t0));
   DGK_HIP(hipEventCreate(&t1));
   DGK_HIP(hipEventRecord(t0, b->stream));
   for (int it = 0; it < nstep; ++it) {
-    if (b->use_graph) DGK_HIP(hipGraphLaunch(b->exec, b->stream));
-    else DGK_TRY(batch_enqueue_step(b));
+    if (b->use_graph) {
+      DGK_HIP(hipGraphLaunch(b->exec[b->next], b->stream));
+      b->next ^= 1;
+    } else {
+      DGK_TRY(batch_enqueue_step(b));
+    }
   }
   DGK_HIP(hipEventRecord(t1, b->stream));
   b->pending.push_back({t0, t1});
@@ -1064,15 +1091,8 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
     }
   }
   if (n) *n = (int)b->groups.size();
-  // the captured step used the previous shapes
-  if (b->exec) {
-    (void)hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
-  }
-  if (b->graph) {
-    (void)hipGraphDestroy(b->graph);
-    b->graph = nullptr;
-  }
+  // the captured steps used the previous shapes
+  batch_release_graphs(b);
   return DGK_OK;
 }
 
