@@ -231,7 +231,7 @@ def test_processing_order_does_not_change_results(form, monkeypatch):
     m = oracle.Mesh(8, 8, 16, p)
     runs = {}
     for env in ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"},
-                {"DGK_ORDER": "identity", "DGK_ALT": "0"}):
+                {"DGK_ORDER": "identity", "DGK_ALT": "0"}, {"DGK_ORDER": "brick:2,2,4"}):
         monkeypatch.delenv("DGK_ORDER", raising=False)
         monkeypatch.delenv("DGK_ALT", raising=False)
         for k, val in env.items():
