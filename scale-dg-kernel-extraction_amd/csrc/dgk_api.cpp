@@ -115,12 +115,18 @@ int refresh_f32(dgk_solver *s, int which) {
   return launch_to_f32(field_ptr(s, which), dst, (int64_t)s->Np * s->Ne, s->stream);
 }
 
+bool alternate_launches() {
+  const char *alt = std::getenv("DGK_ALT");
+  return !(alt && alt[0] == '0');
+}
+
 StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
                      double *out, int mode, double ca, double cb, double dt) {
   StageArgs a;
   a.q = qin; a.q0 = q0; a.u = s->u; a.v = s->v; a.w = s->w; a.out = out;
   a.flux_in = nullptr; a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D; a.Lift = s->Lift;
-  a.ell = s->ell; a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.Ne = s->Ne;
+  a.ell = s->ell; a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = 0;
+  a.Ne = s->Ne;
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
   a.dt_geo = 0;
   a.tile = s->tile;
@@ -133,19 +139,19 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
 int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out,
                  int mode, double ca, double cb, double dt, hipStream_t stream, int dt_geo) {
   if (!stream) stream = s->stream;
-  const int32_t *ord = (s->order_rev && (s->launches++ & 1)) ? s->order_rev : s->order;
+  const int reverse = (s->alternate && (s->launches++ & 1)) ? 1 : 0;
   if (s->evaltype == DGK_EVAL_HIP_MP) {
     StageArgsMP a;
     a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
     a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
-    a.geo = s->geo; a.geoid = s->geoid; a.order = ord; a.Ne = s->Ne;
+    a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = reverse; a.Ne = s->Ne;
     a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
     a.dt_geo = dt_geo;
     a.tile = s->tile;
     return launch_stage_mp(s->Nq, a, stream);
   }
   StageArgs a = stage_args(s, qin, q0, out, mode, ca, cb, dt);
-  a.order = ord;
+  a.reverse = reverse;
   a.dt_geo = dt_geo;
   return launch_stage_f64(s->Nq, a, stream);
 }
@@ -257,13 +263,7 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   // launch starts where the previous one ended, on the fields it touched
   // last, which are still in the memory-side cache (DGK_ALT=0: always
   // forwards).  The arithmetic of an element does not depend on when it runs.
-  if (const char *alt = std::getenv("DGK_ALT"); !(alt && alt[0] == '0')) {
-    std::vector<int32_t> rev((size_t)t.Ne);
-    for (int64_t k = 0; k < t.Ne; ++k)
-      rev[(size_t)k] = t.order.empty() ? (int32_t)(t.Ne - 1 - k) : t.order[(size_t)(t.Ne - 1 - k)];
-    if ((rc = dalloc(s, &s->order_rev, t.Ne)) || (rc = upload(s, s->order_rev, rev)))
-      return fail(rc);
-  }
+  s->alternate = alternate_launches();
   for (int b = 0; b < 3; ++b)
     if (hipMemsetAsync(s->q[b], 0, sizeof(double) * s->nfield, s->stream) != hipSuccess)
       return fail(set_error(DGK_ERR_DEVICE, "hipMemsetAsync failed"));

@@ -88,6 +88,13 @@ __device__ __forceinline__ void nbr_entries(const int32_t *__restrict__ vmapP,
     for (int f = 0; f < 6; ++f) t[f] = nbr[e * 6 + f];
   }
 }
+// Element of launch slot `slot` (< Ne): the processing order, walked
+// forwards or backwards.
+__device__ __forceinline__ int64_t slot_element(int64_t slot, int64_t Ne,
+                                                const int32_t *__restrict__ order, int reverse) {
+  if (reverse) slot = Ne - 1 - slot;
+  return order ? (int64_t)order[slot] : slot;
+}
 template <int NQ>
 __device__ __forceinline__ int64_t nbr_node(bool resolved, int32_t t, int f, int i, int j) {
   return resolved ? (int64_t)t : (int64_t)t * (NQ * NQ * NQ) + nbr_face_node(f, i, j, NQ);
@@ -199,6 +206,7 @@ struct StageArgs {
   const double *geo;
   const uint32_t *geoid;  // nullptr when one record
   const int32_t *order;   // processing order of elements (nullptr: identity)
+  int reverse;            // walk the order backwards
   int64_t Ne;
   double ca, cb, dt;      // RK stage: out = ca*q0 + cb*(q + dt*tend)
   int mode;               // 0 = tendency, 1 = RK stage
@@ -220,6 +228,7 @@ struct StageArgsMP {
   const double *geo;
   const uint32_t *geoid;
   const int32_t *order;
+  int reverse;
   int64_t Ne;
   double ca, cb, dt;
   int mode;

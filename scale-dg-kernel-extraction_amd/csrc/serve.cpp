@@ -315,6 +315,8 @@ int start(dgk_server *sv, Req *r, int64_t off) {
   s.D = op.D; s.Lift = op.Lift; s.ell = op.use_ell ? op.ell : nullptr;
   s.D32 = op.D32; s.ell32 = op.ell32; s.x1D = op.x1D;
   s.axis_aligned = 1;
+  s.alternate = alternate_launches();
+  s.launches = 0;
   s.cur = 0;
 
   const int *ne = r->r.ne;

@@ -36,8 +36,8 @@ struct dgk_solver {
   float *D32 = nullptr, *ell32 = nullptr;
   double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
   int32_t *order = nullptr;  // element processing order
-  int32_t *order_rev = nullptr;  // the same order backwards (every other launch)
-  uint64_t launches = 0;         // stage launches so far (picks the direction)
+  bool alternate = false;    // consecutive stage launches walk the order both ways
+  uint64_t launches = 0;     // stage launches so far (picks the direction)
   int axis_aligned = 0;
   int tile = -1;             // tile-shape candidate chosen for this mesh (-1: process default)
   double *x1D = nullptr;     // LGL nodes (device), for device-side initial conditions
@@ -89,6 +89,8 @@ double *field_ptr(dgk_solver *s, int which);
 int refresh_f32(dgk_solver *s, int which);
 // One stage launch on `stream` (the solver's own stream when nullptr);
 // dt_geo takes the time step from each element's geometry record.
+// Consecutive stage launches alternate direction unless DGK_ALT=0.
+bool alternate_launches();
 int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out,
                  int mode, double ca, double cb, double dt, hipStream_t stream = nullptr,
                  int dt_geo = 0);
