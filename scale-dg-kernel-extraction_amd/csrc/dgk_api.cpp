@@ -126,7 +126,7 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
   a.q = qin; a.q0 = q0; a.u = s->u; a.v = s->v; a.w = s->w; a.out = out;
   a.flux_in = nullptr; a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D; a.Lift = s->Lift;
   a.ell = s->ell; a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = 0;
-  a.Ne = s->Ne;
+  a.zplane = s->zplane; a.Ne = s->Ne;
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
   a.dt_geo = 0;
   a.tile = s->tile;
@@ -144,7 +144,8 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
     StageArgsMP a;
     a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
     a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
-    a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = reverse; a.Ne = s->Ne;
+    a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = reverse;
+    a.zplane = s->zplane; a.Ne = s->Ne;
     a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
     a.dt_geo = dt_geo;
     a.tile = s->tile;
@@ -264,6 +265,7 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   // last, which are still in the memory-side cache (DGK_ALT=0: always
   // forwards).  The arithmetic of an element does not depend on when it runs.
   s->alternate = alternate_launches();
+  s->zplane = t.zplane;
   for (int b = 0; b < 3; ++b)
     if (hipMemsetAsync(s->q[b], 0, sizeof(double) * s->nfield, s->stream) != hipSuccess)
       return fail(set_error(DGK_ERR_DEVICE, "hipMemsetAsync failed"));

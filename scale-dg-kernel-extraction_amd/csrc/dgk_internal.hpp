@@ -89,11 +89,20 @@ __device__ __forceinline__ void nbr_entries(const int32_t *__restrict__ vmapP,
   }
 }
 // Element of launch slot `slot` (< Ne): the processing order, walked
-// forwards or backwards.
+// forwards or backwards.  Without an order array, zplane > 0 (planes of
+// zplane elements, a multiple of 8 of them) gives the XCD that runs slots
+// [x Ne/8, (x+1) Ne/8) the planes x, x+8, x+16, ...
 __device__ __forceinline__ int64_t slot_element(int64_t slot, int64_t Ne,
-                                                const int32_t *__restrict__ order, int reverse) {
+                                                const int32_t *__restrict__ order, int reverse,
+                                                int64_t zplane) {
   if (reverse) slot = Ne - 1 - slot;
-  return order ? (int64_t)order[slot] : slot;
+  if (order) return (int64_t)order[slot];
+  if (zplane > 0) {
+    const uint32_t per = (uint32_t)(Ne >> 3), P = (uint32_t)zplane, s = (uint32_t)slot;
+    const uint32_t x = s / per, t = s - x * per, r = t / P;
+    return (int64_t)(x + 8u * r) * P + (t - r * P);
+  }
+  return slot;
 }
 template <int NQ>
 __device__ __forceinline__ int64_t nbr_node(bool resolved, int32_t t, int f, int i, int j) {
@@ -116,6 +125,7 @@ struct MeshTables {
   std::vector<double> D;          // Nq*Nq column-major
   std::vector<double> Lift;       // Nq^3*6 column-major
   std::vector<int32_t> order;     // element processing order (empty: identity)
+  int64_t zplane = 0;             // > 0: z-plane-interleaved order of planes of this size
   bool axis_aligned = false;      // every face normal lies along its face axis
 };
 
@@ -142,8 +152,7 @@ std::vector<int32_t> brick_order(int NeX, int NeY, int NeZ, int bx, int by, int 
 // the assignment of elements to workgroups changes, never the arithmetic.
 // DGK_ORDER=identity keeps the element numbering.
 int64_t z_plane_size(const MeshTables &t);  // P, or 0 when not of that form
-std::vector<int32_t> zplane_interleaved_order(int64_t P, int64_t NeZ);
-void default_order(MeshTables &t);
+void default_order(MeshTables &t);           // sets zplane (the kernels compute the order)
 
 int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
                                 int64_t NhaloNode, const double *D1D,
@@ -207,6 +216,7 @@ struct StageArgs {
   const uint32_t *geoid;  // nullptr when one record
   const int32_t *order;   // processing order of elements (nullptr: identity)
   int reverse;            // walk the order backwards
+  int64_t zplane;         // order == nullptr and > 0: z-plane-interleaved order
   int64_t Ne;
   double ca, cb, dt;      // RK stage: out = ca*q0 + cb*(q + dt*tend)
   int mode;               // 0 = tendency, 1 = RK stage
@@ -229,6 +239,7 @@ struct StageArgsMP {
   const uint32_t *geoid;
   const int32_t *order;
   int reverse;
+  int64_t zplane;
   int64_t Ne;
   double ca, cb, dt;
   int mode;

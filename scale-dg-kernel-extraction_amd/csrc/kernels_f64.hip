@@ -94,7 +94,8 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
           const double *__restrict__ ell_, const double *__restrict__ geo_,
           const uint32_t *__restrict__ geoid_, const int32_t *__restrict__ order_,
           const int64_t Ne_, const double ca_, const double cb_, const double dt_,
-          const int mode_, const int axis_aligned_, const int dt_geo_, const int rev_) {
+          const int mode_, const int axis_aligned_, const int dt_geo_, const int rev_,
+          const int64_t zp_) {
   // Read-only operands are __restrict__ kernel parameters, so wave-uniform
   // reads of them (operators, geometry, neighbour table) are scalar loads.
   using C = Staged<NQ, VREG>;
@@ -119,11 +120,11 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   if (EPB == 1) {
     // one element per workgroup: its index, neighbour table entries and
     // geometry record are wave-uniform and read through the scalar cache
-    e = slot < Ne_ ? slot_element(slot, Ne_, order_, rev_) : 0;
+    e = slot < Ne_ ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
     e = uniform(e);
     g = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   } else {
-    e = active ? slot_element(slot, Ne_, order_, rev_) : 0;
+    e = active ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
     g = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   }
   const int64_t base = e * NP + ij;
@@ -154,7 +155,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
         const int del = d / NP, dn = d - del * NP;
         const int64_t sl = slot0 + del;
         int64_t off = 0;
-        if (del < EPB && sl < Ne_) off = slot_element(sl, Ne_, order_, rev_) * NP + dn;
+        if (del < EPB && sl < Ne_) off = slot_element(sl, Ne_, order_, rev_, zp_) * NP + dn;
 #pragma unroll
         for (int fld = 0; fld < C::NSTAGED; ++fld) {
           const char *gp = (const char *)(src[fld] + off) + (C::VEC16 ? 0 : 4 * (lane & 1));
@@ -346,7 +347,7 @@ stage_f64_lift(const StageArgs P) {
   const int i = ij % NQ, j = ij / NQ;
   const int64_t slot = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB + el;
   const bool active = slot < P.Ne;
-  const int64_t e = active ? slot_element(slot, P.Ne, P.order, P.reverse) : 0;
+  const int64_t e = active ? slot_element(slot, P.Ne, P.order, P.reverse, P.zplane) : 0;
 
   const double *g = P.geo + (active && P.geoid ? (int64_t)P.geoid[e] * kGeoStride : 0);
   const int64_t base = e * NP + ij;
@@ -488,7 +489,7 @@ int launch_nq(const StageArgs &a, hipStream_t s, bool vreg) {
     hipLaunchKernelGGL((stage_f64<NQ, true>), dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
                        a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
                        a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo,
-                       a.reverse);
+                       a.reverse, a.zplane);
   } else if (a.ell) {
     using T = Staged<NQ, false>;
     const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
@@ -497,7 +498,7 @@ int launch_nq(const StageArgs &a, hipStream_t s, bool vreg) {
     hipLaunchKernelGGL((stage_f64<NQ, false>), dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
                        a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
                        a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo,
-                       a.reverse);
+                       a.reverse, a.zplane);
   } else {
     using T = Tile<NQ>;
     const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;

@@ -206,7 +206,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
            const double *__restrict__ geo_, const uint32_t *__restrict__ geoid_,
            const int32_t *__restrict__ order_, const int64_t Ne_, const double ca_,
            const double cb_, const double dt_, const int mode_, const int axis_aligned_,
-           const int dt_geo_, const int rev_) {
+           const int dt_geo_, const int rev_, const int64_t zp_) {
   using C = MP<P, NQ, EPB, SEQ>;
   using vel_t = typename P::vel_t;
   using op_t = typename P::op_t;
@@ -227,11 +227,11 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   int64_t e;
   const double *g;
   if (EPB == 1) {
-    e = slot < Ne_ ? slot_element(slot, Ne_, order_, rev_) : 0;
+    e = slot < Ne_ ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
     e = uniform_mp(e);
     g = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   } else {
-    e = active ? slot_element(slot, Ne_, order_, rev_) : 0;
+    e = active ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
     g = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   }
   const int64_t ebase = e * NP;
@@ -535,7 +535,7 @@ int launch_shape(const Args &a, hipStream_t s) {
   hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ>), dim3((unsigned)nblk), dim3(T::THREADS),
                      T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr, a.vmapP, a.D,
                      a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode,
-                     a.axis_aligned, a.dt_geo, a.reverse);
+                     a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }

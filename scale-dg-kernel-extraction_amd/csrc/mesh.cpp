@@ -117,24 +117,16 @@ int64_t z_plane_size(const MeshTables &t) {
   return P;
 }
 
-std::vector<int32_t> zplane_interleaved_order(int64_t P, int64_t NeZ) {
-  std::vector<int32_t> order;
-  order.reserve((size_t)(P * NeZ));
-  for (int x = 0; x < 8; ++x)
-    for (int64_t p = x; p < NeZ; p += 8)
-      for (int64_t o = 0; o < P; ++o) order.push_back((int32_t)(p * P + o));
-  return order;
-}
-
 void default_order(MeshTables &t) {
   t.order.clear();
+  t.zplane = 0;
   const char *env = std::getenv("DGK_ORDER");
   if (env && std::string(env) == "identity") return;
   // only where the planes split evenly over the XCDs: otherwise the XCDs'
   // slot ranges cut across the plane groups (measured 9% slower at 21^3)
   const int64_t P = z_plane_size(t);
   if (P == 0 || (t.Ne / P) % 8 != 0) return;
-  t.order = zplane_interleaved_order(P, t.Ne / P);
+  t.zplane = P;  // the kernels compute this order; no array
 }
 
 int load_operators(const char *dir, int p, std::vector<double> &x1D,
@@ -205,7 +197,10 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
   const char *env = std::getenv("DGK_ORDER");
   if (env && std::sscanf(env, "brick:%d,%d,%d", &bx, &by, &bz) == 3 && bx > 0 && by > 0 &&
       bz > 0)
+  {
     t.order = brick_order(NeX, NeY, NeZ, bx, by, bz);
+    t.zplane = 0;
+  }
   else
     default_order(t);
   t.axis_aligned = geo_axis_aligned(t.geo);
@@ -260,6 +255,7 @@ int build_tables_batch(int p, const double *D1D, const double *Lift_mat,
         }
   }
   t.order.clear();
+  t.zplane = 0;
   t.axis_aligned = true;
   return DGK_OK;
 }
