@@ -84,7 +84,14 @@ struct Staged {
   static constexpr int MIN_WAVES_EU = (VREG && NQ == 12) ? 3 : 1;
 };
 
-template <int NQ, bool VREG>
+// AXAL: every face normal lies along its face's axis (checked on the host), so
+// only q and the normal velocity component of the neighbour are gathered (the
+// other components enter the normal velocity as exact zeros).  The general
+// form holds three more traces per face in registers; a compile-time choice
+// lets the axis-aligned kernel drop them (p = 1: 86 -> 62 VGPRs, 5 -> 8 waves
+// per SIMD).  The register form at p = 11 keeps the test at run time: with
+// it decided at compile time the allocation lands at 168 VGPRs with 2 spilled.
+template <int NQ, bool VREG, bool AXAL>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, VREG>::THREADS),
                                amdgpu_waves_per_eu(Staged<NQ, VREG>::MIN_WAVES_EU)))
 stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
@@ -173,12 +180,11 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   }
 
   // neighbour traces of face node (i,j) on all six faces, and the RK base
-  // column, in flight together with the async copy
-  // With axis-aligned face normals only the normal velocity component of the
-  // neighbour is gathered (the others enter the normal velocity as exact
-  // zeros); the component is fixed per face at compile time.
+  // column, in flight together with the async copy (axis-aligned: q and the
+  // normal velocity component, fixed per face at compile time)
+  constexpr bool AX_RUNTIME = VREG && NQ == 12;
+  const bool axal = AX_RUNTIME ? axis_aligned_ != 0 : AXAL;
   double qP[6], uP[6], vP[6], wP[6], q0v[NQ];
-  const bool axal = axis_aligned_ != 0;
   if (active && !flux_in_) {
     auto nbr_at = [&](int f) -> int64_t {
       if constexpr (EPB > 1)
@@ -458,13 +464,13 @@ stage_f64_lift(const StageArgs P) {
   }
 }
 
-template <int NQ, bool VREG>
+template <int NQ, bool VREG, bool AXAL>
 int prepare_shape_valu() {
   static int prepared_dev = -1;
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG, AXAL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)Staged<NQ, VREG>::LDS_BYTES));
     prepared_dev = dev;
@@ -474,37 +480,39 @@ int prepare_shape_valu() {
 
 template <int NQ>
 int prepare_nq() {
-  DGK_TRY((prepare_shape_valu<NQ, false>()));
-  return prepare_shape_valu<NQ, true>();
+  DGK_TRY((prepare_shape_valu<NQ, false, false>()));
+  DGK_TRY((prepare_shape_valu<NQ, false, true>()));
+  DGK_TRY((prepare_shape_valu<NQ, true, false>()));
+  return prepare_shape_valu<NQ, true, true>();
 }
 
-// VALU stage kernel (VREG: v, w in registers), or the stored-Lift kernel.
+template <int NQ, bool VREG, bool AXAL>
+int launch_valu(const StageArgs &a, hipStream_t s) {
+  using T = Staged<NQ, VREG>;
+  const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
+  if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
+  DGK_TRY((prepare_shape_valu<NQ, VREG, AXAL>()));
+  hipLaunchKernelGGL((stage_f64<NQ, VREG, AXAL>), dim3((unsigned)nblk), dim3(T::THREADS),
+                     T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr,
+                     a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt,
+                     a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
+  DGK_HIP(hipGetLastError());
+  return DGK_OK;
+}
+
+// VALU stage kernel (VREG: v, w in registers; the axis-aligned or general
+// normal form), or the stored-Lift kernel.
 template <int NQ>
 int launch_nq(const StageArgs &a, hipStream_t s, bool vreg) {
-  if (a.ell && vreg) {
-    using T = Staged<NQ, true>;
-    const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
-    if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-    DGK_TRY((prepare_shape_valu<NQ, true>()));
-    hipLaunchKernelGGL((stage_f64<NQ, true>), dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
-                       a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
-                       a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo,
-                       a.reverse, a.zplane);
-  } else if (a.ell) {
-    using T = Staged<NQ, false>;
-    const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
-    if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-    DGK_TRY((prepare_shape_valu<NQ, false>()));
-    hipLaunchKernelGGL((stage_f64<NQ, false>), dim3((unsigned)nblk), dim3(T::THREADS), T::LDS_BYTES, s,
-                       a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr, a.vmapP, a.D, a.ell,
-                       a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo,
-                       a.reverse, a.zplane);
-  } else {
-    using T = Tile<NQ>;
-    const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
-    if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-    hipLaunchKernelGGL(stage_f64_lift<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), 0, s, a);
+  if (a.ell) {
+    const bool ax = a.axis_aligned != 0;
+    if (vreg) return ax ? launch_valu<NQ, true, true>(a, s) : launch_valu<NQ, true, false>(a, s);
+    return ax ? launch_valu<NQ, false, true>(a, s) : launch_valu<NQ, false, false>(a, s);
   }
+  using T = Tile<NQ>;
+  const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
+  if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
+  hipLaunchKernelGGL(stage_f64_lift<NQ>, dim3((unsigned)nblk), dim3(T::THREADS), 0, s, a);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }

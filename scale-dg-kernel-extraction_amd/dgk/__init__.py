@@ -155,14 +155,17 @@ def tile_shapes(evaltype, p: int) -> list:
     return e[:n.value].tolist()
 
 
-def kernel_name(evaltype, p: int, cand: int) -> str:
-    """Name of the stage kernel that tile candidate `cand` launches."""
+def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True) -> str:
+    """Name of the stage kernel that tile candidate `cand` launches (the FP64
+    VALU kernels come in an axis-aligned and a general face-normal form; the
+    structured meshes are axis-aligned)."""
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
     nq = p + 1
     nvalu = 2 if evaltype == EVAL_HIP and nq <= 16 else 0  # FP64 VALU kernels first
     if cand < nvalu:
-        return f"stage_f64<{nq}, {'true' if cand == 1 else 'false'}>"
+        b = lambda x: "true" if x else "false"
+        return f"stage_f64<{nq}, {b(cand == 1)}, {b(axis_aligned)}>"
     shapes = tile_shapes(evaltype, p)
     mfma = shapes[nvalu:]
     mc = cand - nvalu

@@ -249,3 +249,48 @@ def test_processing_order_does_not_change_results(form, monkeypatch):
     if form == "HIP":
         want, _, _ = m.run(q, u, v, w, dt, nstep)
         assert maxrel(ref, want) <= LOOP_TOL
+
+
+def tilted_normals(m):
+    """normal_fn with off-axis components, constant over each face and
+    differing between elements (several geometry records), so the stage
+    kernels take their general normal-velocity form
+    (VelM = u n1 + v n2 + w n3, mod_advect3d_eq.f90:121-127)."""
+    nf = m.normal_fn.copy()
+    Nfp = m.Nfp
+    for e in range(m.Ne):
+        for f in range(6):
+            seg = slice(f * Nfp, (f + 1) * Nfp)
+            a = int(np.argmax(np.abs(nf[:, e, f * Nfp])))
+            nf[(a + 1) % 3, e, seg] = 0.3 + 0.05 * (e % 3)
+            nf[(a + 2) % 3, e, seg] = -0.2
+    return nf
+
+
+@pytest.mark.parametrize("p", [1, 3, 5, 7, 8, 12, 15])
+def test_general_face_normals_every_tile_shape(p):
+    """Face normals off the face axes through the reference-array constructor:
+    every FP64 tile shape against the oracle's tendency and time loop, and
+    the mixed-precision form to its relative-L2 bar."""
+    ne = (3, 2, 2) if p <= 7 else (2, 1, 2)
+    m = oracle.Mesh(*ne, p)
+    m.normal_fn = tilted_normals(m)
+    q, u, v, w = inputs.random_fields(*ne, p, seed=1300 + p, vel_scale=0.8)
+    dt = 0.2 * inputs.cfl_dt(inputs.lgl_nodes(p), ne, (0.8, 0.8, 0.8))
+    tend_ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    loop_ref, _, _ = m.run(q, u, v, w, dt, 3)
+    try:
+        for cand in range(len(dgk.tile_shapes("HIP", p))):
+            dgk.tile_select("HIP", p, cand)
+            with solver_from_mesh(m) as s:
+                s.set_fields(q, u, v, w)
+                assert maxrel(s.cal_tend(), tend_ref) <= TEND_TOL, cand
+                s.step(dt, 3)
+                assert maxrel(s.get_field(dgk.FIELD_Q), loop_ref) <= LOOP_TOL, cand
+    finally:
+        dgk.tile_select("HIP", p, -1)
+    with solver_from_mesh(m, "HIP_MP") as s:
+        s.set_fields(q, u, v, w)
+        got = s.cal_tend()
+    rel_l2 = float(np.linalg.norm(got - tend_ref) / np.linalg.norm(tend_ref))
+    assert rel_l2 <= 1e-5
