@@ -119,6 +119,8 @@ struct MP {
   // Register budget: the mixed exchange shapes keep 4 waves per SIMD (at most
   // 128 registers; left free, p = 7 lands two above and drops to 3)
   static constexpr int MIN_WAVES_EU = (sizeof(typename P::op_t) == 4 && XCH) ? 4 : 1;
+  // Shapes whose face-normal form is tested at run time (see stage_mfma).
+  static constexpr bool AX_RUNTIME = SEQ || MIN_WAVES_EU > 1;
   static constexpr int JOBS = NAREA * MTILES;               // (direction, tile) GEMMs per pass
   static constexpr int JPW = (JOBS + WAVES - 1) / WAVES;
 };
@@ -195,7 +197,15 @@ __device__ __forceinline__ void mfma_tile(typename P::op_t *tile,
   }
 }
 
-template <class P, int NQ, int EPB, bool SEQ>
+// AXAL: every face normal lies along its face's axis (checked on the host); a
+// compile-time form, so the axis-aligned kernel holds no registers for the
+// general normal velocity (one more wave per SIMD for several multi-wavefront
+// shapes, e.g. the mixed p = 10, 11, 15 defaults).  Two kinds of shape keep
+// the test at run time (AX_RUNTIME), because decided at compile time their
+// allocations grow: the direction-by-direction (SEQ) shapes (p = 5..9:
+// 5 -> 4 waves per SIMD) and the mixed one-wavefront shapes held to 4 waves
+// (p = 2..4 then spill 2 to 24 VGPRs).
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, MP<P, NQ, EPB, SEQ>::THREADS),
                                amdgpu_waves_per_eu(MP<P, NQ, EPB, SEQ>::MIN_WAVES_EU)))
 stage_mfma(const double *__restrict__ q_, const double *q0_,
@@ -238,7 +248,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   const int64_t base = ebase + ij;
   const bool rk = mode_ == 1;
   const bool use_q0 = rk && ca_ != 0.0;
-  const bool axal = axis_aligned_ != 0;
+  const bool axal = C::AX_RUNTIME ? axis_aligned_ != 0 : AXAL;
   const int R = el * NQ2;
   const vel_t *const velp[3] = {u_, v_, w_};
 
@@ -512,13 +522,13 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   }
 }
 
-template <class P, int NQ, int EPB, bool SEQ>
-int prepare_shape() {
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL>
+int prepare_form() {
   static int prepared_dev = -1;
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ, AXAL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)MP<P, NQ, EPB, SEQ>::LDS_BYTES));
     prepared_dev = dev;
@@ -526,18 +536,34 @@ int prepare_shape() {
   return DGK_OK;
 }
 
-template <class P, int NQ, int EPB, bool SEQ, class Args>
-int launch_shape(const Args &a, hipStream_t s) {
+// Both face-normal forms of a shape (one, testing at run time, for the
+// AX_RUNTIME shapes).
+template <class P, int NQ, int EPB, bool SEQ>
+int prepare_shape() {
+  DGK_TRY((prepare_form<P, NQ, EPB, SEQ, false>()));
+  if constexpr (!MP<P, NQ, EPB, SEQ>::AX_RUNTIME) DGK_TRY((prepare_form<P, NQ, EPB, SEQ, true>()));
+  return DGK_OK;
+}
+
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, class Args>
+int launch_form(const Args &a, hipStream_t s) {
   using T = MP<P, NQ, EPB, SEQ>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  DGK_TRY((prepare_shape<P, NQ, EPB, SEQ>()));
-  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ>), dim3((unsigned)nblk), dim3(T::THREADS),
-                     T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr, a.vmapP, a.D,
-                     a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt, a.mode,
-                     a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
+  DGK_TRY((prepare_form<P, NQ, EPB, SEQ, AXAL>()));
+  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ, AXAL>), dim3((unsigned)nblk),
+                     dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr,
+                     a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt,
+                     a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
+}
+
+template <class P, int NQ, int EPB, bool SEQ, class Args>
+int launch_shape(const Args &a, hipStream_t s) {
+  if constexpr (!MP<P, NQ, EPB, SEQ>::AX_RUNTIME)
+    if (a.axis_aligned) return launch_form<P, NQ, EPB, SEQ, true>(a, s);
+  return launch_form<P, NQ, EPB, SEQ, false>(a, s);
 }
 
 // Candidate c of order NQ (compile-time), as a kernel instantiation.
