@@ -169,9 +169,15 @@ def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True) -> str:
     shapes = tile_shapes(evaltype, p)
     mfma = shapes[nvalu:]
     mc = cand - nvalu
-    seq = nq > 16 or (mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or mfma[mc] == 1))
+    epb = mfma[mc]
+    seq = nq > 16 or (mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or epb == 1))
     prec = "PrecF64" if evaltype == EVAL_HIP else "PrecMixed"
-    return f"stage_mfma<{prec}, {nq}, {mfma[mc]}, {'true' if seq else 'false'}>"
+    # the direction-by-direction shapes and the mixed one-wavefront shapes
+    # test the face normals at run time (instantiated as the general form)
+    one_wave = epb * nq * nq <= 64
+    ax_runtime = seq or (evaltype != EVAL_HIP and one_wave)
+    b = lambda x: "true" if x else "false"
+    return f"stage_mfma<{prec}, {nq}, {epb}, {b(seq)}, {b(axis_aligned and not ax_runtime)}>"
 
 
 def tile_select(evaltype, p: int, cand: int) -> None:

@@ -190,10 +190,13 @@ def test_shapes_above_the_operator_files():
         assert dgk.tile_shapes("HIP", p) == [1]
         assert dgk.tile_shapes("HIP_MP", p) == [1]
         assert dgk.tile_selected("HIP", p) == 0
-    assert dgk.kernel_name("HIP", 17, 0) == "stage_mfma<PrecF64, 18, 1, true>"
+    assert dgk.kernel_name("HIP", 17, 0) == "stage_mfma<PrecF64, 18, 1, true, false>"
     assert dgk.kernel_name("HIP", 7, 0) == "stage_f64<8, false, true>"
     assert dgk.kernel_name("HIP", 7, 1) == "stage_f64<8, true, true>"
     assert dgk.kernel_name("HIP", 7, 1, axis_aligned=False) == "stage_f64<8, true, false>"
-    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true>"
-    assert dgk.kernel_name("HIP_MP", 15, 2) == "stage_mfma<PrecMixed, 16, 1, true>"
+    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true, false>"
+    assert dgk.kernel_name("HIP", 7, 2) == "stage_mfma<PrecF64, 8, 1, false, true>"
+    assert dgk.kernel_name("HIP_MP", 7, 0) == "stage_mfma<PrecMixed, 8, 1, false, false>"
+    assert dgk.kernel_name("HIP_MP", 11, 0) == "stage_mfma<PrecMixed, 12, 1, false, true>"
+    assert dgk.kernel_name("HIP_MP", 15, 2) == "stage_mfma<PrecMixed, 16, 1, true, false>"
     assert dgk.element_sizes(19) == dict(Nq=20, Np=8000, Nfp=400, NfpTot=2400)
