@@ -88,7 +88,7 @@ struct Staged {
 // only q and the normal velocity component of the neighbour are gathered (the
 // other components enter the normal velocity as exact zeros).  The general
 // form holds three more traces per face in registers; a compile-time choice
-// lets the axis-aligned kernel drop them (p = 1: 86 -> 62 VGPRs, 5 -> 8 waves
+// lets the axis-aligned kernel drop them (p = 1: 85 -> 62 VGPRs, 5 -> 8 waves
 // per SIMD).  The register form at p = 11 keeps the test at run time: with
 // it decided at compile time the allocation lands at 168 VGPRs with 2 spilled.
 template <int NQ, bool VREG, bool AXAL>
