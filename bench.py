@@ -67,6 +67,10 @@ def parse():
                     help="arrival rate of the serving stream, requests/s")
     ap.add_argument("--serve-burst", type=int, default=60,
                     help="requests submitted at once to measure serving capacity (0: skip)")
+    ap.add_argument("--per-order-only", type=int, default=0, metavar="P",
+                    help="only run --steps steps of the per-order workload at order P "
+                         "(periodic box of about 1.7e7 DOF, default tile shape, --form) "
+                         "for counter profiles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (the reference "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -250,6 +254,25 @@ def flops_per_dof_update(p):
     return 3.0 * (6 * nq + 25 + 120.0 / nq)
 
 
+def per_order_box(p):
+    """Elements per axis and DOF of the per-order workload at order p."""
+    np_ = (p + 1) ** 3
+    ne = max(2, round((16_777_216 / np_) ** (1.0 / 3.0)))
+    return ne, ne ** 3 * np_
+
+
+def per_order_only(dgk, form, p, steps):
+    """`steps` time steps of the per-order workload at order p, nothing else
+    (the command that rocprofv3 counter passes wrap)."""
+    ne, dof = per_order_box(p)
+    with dgk.Solver.structured(form, ne, ne, ne, p) as s:
+        s.init_gaussian((0.3, 0.6, 0.2), 0.1, (0.9, -0.4, 0.3))
+        s.step(0.05 * (2.0 / ne) / (p + 1), steps)
+        s.synchronize()
+    print(json.dumps({"per_order_only": p, "form": form, "ne": ne, "dof": dof, "steps": steps,
+                      "kernel": dgk.kernel_name(form, p, dgk.tile_selected(form, p))}), flush=True)
+
+
 def per_order_leg(dgk, orders=(1, 3, 5, 7, 9, 11, 13, 15), steps=20):
     """Stage-kernel rate of both forms at every other order on a periodic box
     of about 1.68e7 DOF (Gaussian blob, constant velocity), default tile
@@ -258,8 +281,7 @@ def per_order_leg(dgk, orders=(1, 3, 5, 7, 9, 11, 13, 15), steps=20):
     out = []
     for p in orders:
         np_ = (p + 1) ** 3
-        ne = max(2, round((16_777_216 / np_) ** (1.0 / 3.0)))
-        dof = ne ** 3 * np_
+        ne, dof = per_order_box(p)
         for form in ("HIP", "HIP_MP"):
             with dgk.Solver.structured(form, ne, ne, ne, p) as s:
                 s.init_gaussian((0.3, 0.6, 0.2), 0.1, (0.9, -0.4, 0.3))
@@ -364,6 +386,9 @@ def main():
                                    "described in DESIGN.md, not provided as a run mode"}))
         sys.exit(2)
     import dgk
+    if args.per_order_only:
+        per_order_only(dgk, args.form, args.per_order_only, args.steps)
+        return
     from dgk import inputs
 
     wl, r = time_solver(dgk, inputs, args.form, args.steps, args.warmup,
