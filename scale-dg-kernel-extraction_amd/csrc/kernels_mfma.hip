@@ -576,11 +576,13 @@ struct Cand {
 
 // Tile-shape candidate selected per Nq for the mixed form.  Defaults from
 // per-order timings on MI355X at about 1.7e7 DOF (whole-step runs, every
-// candidate): the one-wavefront shape, except the direction-by-direction
-// shape at p = 2 and p = 14.
-constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 0, 3, 0, 0, 0, 0, 0, 0, 0,
+// candidate): the two-wavefront shape at p = 1..4 (its axis-aligned form
+// holds fewer registers than the one-wavefront shape, which keeps the
+// run-time test), the one-wavefront shape above, and the direction-by-
+// direction shape at p = 14.
+constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 1, 1, 1, 1, 0, 0, 0, 0, 0,
                                         0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
-int g_mp_shape[kMaxNq + 1] = {0, 0, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
+int g_mp_shape[kMaxNq + 1] = {0, 0, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
 
 template <class P, int NQ>
 int prepare_nq_all() {

@@ -165,14 +165,14 @@ def test_tile_shapes_host_side():
     assert dgk.tile_selected("HIP_MP", 7) == 2
     dgk.tile_select("HIP_MP", 7, -1)
     assert dgk.tile_selected("HIP_MP", 7) == 0
-    # mixed defaults: one wavefront, except direction by direction at p = 2
-    # and 14
+    # mixed defaults: two wavefronts at p = 1..4, one above, direction by
+    # direction at p = 14
     assert [dgk.tile_selected("HIP_MP", p) for p in range(1, 16)] == (
-        [0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
+        [1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
     dgk.tile_select("HIP_MP", 2, 0)
     assert dgk.tile_selected("HIP_MP", 2) == 0
     dgk.tile_select("HIP_MP", 2, -1)
-    assert dgk.tile_selected("HIP_MP", 2) == 3
+    assert dgk.tile_selected("HIP_MP", 2) == 1
     # FP64 defaults: the register VALU kernel at p = 3 and 6..13, the
     # direction-by-direction MFMA kernel (the last candidate) at p = 14, 15,
     # the staged VALU kernel else
