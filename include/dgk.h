@@ -233,8 +233,9 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  *           elements, then one wavefront processed direction by direction
  *           (a third of the LDS), for p <= 7; for p >= 8 one element, two
  *           elements, and one element direction by direction; above p = 15
- *           only the last.  Defaults: candidate 0, except the direction-by-
- *           direction shape at p = 2 and 14.
+ *           only the last.  Defaults: two wavefronts (candidate 1) at
+ *           p = 1..4, the direction-by-direction shape at p = 14,
+ *           candidate 0 otherwise.
  *   HIP:    candidate 0 is the VALU kernel with q, u, v, w staged in LDS,
  *           1 the VALU kernel with v, w in registers (half the LDS);
  *           candidates 2.. are the FP64 MFMA kernel's shapes as for HIP_MP,
