@@ -60,6 +60,28 @@ struct Timed {
   }
 };
 
+// Fold the event pairs that have completed, oldest first, without waiting;
+// keeps the pending list short for callers that step for a long time
+// without synchronising.
+int poll_timers(dgk_solver *s) {
+  size_t done = 0;
+  for (; done < s->pending.size(); ++done) {
+    const auto &p = s->pending[done];
+    const hipError_t q = hipEventQuery(p.b);
+    if (q == hipErrorNotReady) break;
+    if (q != hipSuccess) return set_error(DGK_ERR_DEVICE, std::string("hipEventQuery: ") +
+                                                              hipGetErrorString(q));
+    float ms = 0.f;
+    DGK_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    s->sec[p.slot] += 1e-3 * (double)ms;
+    s->cnt[p.slot] += 1;
+    s->free_events.push_back(p.a);
+    s->free_events.push_back(p.b);
+  }
+  s->pending.erase(s->pending.begin(), s->pending.begin() + (std::ptrdiff_t)done);
+  return DGK_OK;
+}
+
 int resolve_timers(dgk_solver *s) {
   if (s->pending.empty()) return DGK_OK;
   DGK_HIP(hipStreamSynchronize(s->stream));
@@ -553,6 +575,7 @@ int dgk_step(dgk_solver *s, double dt, int nstep) {
   // A->B, stage 2 (B, q0=A)->C, stage 3 (C, q0=A)->A.  Stage 3 overwrites
   // A node-for-node after reading q0 there, and no element reads another
   // element's q0, so no copy of q0 is needed.
+  DGK_TRY(poll_timers(s));
   double *A = s->q[s->cur], *B = s->q[(s->cur + 1) % 3], *C = s->q[(s->cur + 2) % 3];
   Timed main_tm(s, T_MAIN, true);
   for (int it = 0; it < nstep; ++it) {
@@ -581,7 +604,7 @@ int dgk_minmax(dgk_solver *s, double *qmin, double *qmax) {
   DGK_HIP(hipStreamSynchronize(s->stream));
   if (qmin) *qmin = r[0];
   if (qmax) *qmax = r[1];
-  return DGK_OK;
+  return resolve_timers(s);  // the stream is idle: folding costs no wait
 }
 
 int dgk_synchronize(dgk_solver *s) {
@@ -710,6 +733,30 @@ int batch_capture(dgk_batch *b) {
     }
   }
   b->next = 0;
+  return DGK_OK;
+}
+
+// Fold the batch's completed timing pairs, oldest first; with wait, all of
+// them after the batch stream has drained.
+int batch_fold_timers(dgk_batch *b, bool wait) {
+  if (wait) DGK_HIP(hipStreamSynchronize(b->stream));
+  size_t done = 0;
+  for (; done < b->pending.size(); ++done) {
+    auto &p = b->pending[done];
+    if (!wait) {
+      const hipError_t q = hipEventQuery(p.second);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) return set_error(DGK_ERR_DEVICE, std::string("hipEventQuery: ") +
+                                                                hipGetErrorString(q));
+    }
+    float ms = 0.f;
+    DGK_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    b->step_seconds += 1e-3 * ms;
+    b->step_calls += 1;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  b->pending.erase(b->pending.begin(), b->pending.begin() + (std::ptrdiff_t)done);
   return DGK_OK;
 }
 
@@ -845,6 +892,8 @@ int dgk_batch_set_field(dgk_batch *b, int inst, int which, const double *f, int6
   dgk_solver *s = b->groups[l->group];
   const int64_t off = l->elem_off * s->Np, cnt = l->Ne * s->Np;
   if (n != cnt) return set_error(DGK_ERR_ARG, "field length must be Np*Ne of the instance");
+  // a replayed step runs on the batch stream: let it finish before overwriting
+  DGK_HIP(hipStreamSynchronize(b->stream));
   double *dst = field_ptr(s, which) + off;
   DGK_HIP(hipMemcpyAsync(dst, f, sizeof(double) * cnt, hipMemcpyHostToDevice, s->stream));
   if (which != DGK_FIELD_Q && s->evaltype == DGK_EVAL_HIP_MP) {
@@ -908,6 +957,7 @@ int dgk_batch_step(dgk_batch *b, int nstep) {
   // fields uploaded on the group streams are ordered before the step
   for (dgk_solver *s : b->groups) DGK_HIP(hipStreamSynchronize(s->stream));
   if (b->use_graph && !b->exec[0]) DGK_TRY(batch_capture(b));
+  DGK_TRY(batch_fold_timers(b, false));
   hipEvent_t t0 = nullptr, t1 = nullptr;
   DGK_HIP(hipEventCreate(&t0));
   DGK_HIP(hipEventCreate(&t1));
@@ -927,17 +977,7 @@ int dgk_batch_step(dgk_batch *b, int nstep) {
 
 int dgk_batch_synchronize(dgk_batch *b) {
   DGK_TRY(batch_enter(b));
-  DGK_HIP(hipStreamSynchronize(b->stream));
-  for (auto &p : b->pending) {
-    float ms = 0.f;
-    DGK_HIP(hipEventElapsedTime(&ms, p.first, p.second));
-    b->step_seconds += 1e-3 * ms;
-    b->step_calls += 1;
-    (void)hipEventDestroy(p.first);
-    (void)hipEventDestroy(p.second);
-  }
-  b->pending.clear();
-  return DGK_OK;
+  return batch_fold_timers(b, true);
 }
 
 int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls) {
@@ -1078,6 +1118,9 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
                        double *us_default, int max, int *n) {
   DGK_TRY(batch_enter(b));
   if (reps < 1) return set_error(DGK_ERR_ARG, "reps must be >= 1");
+  // the timing launches write a rotation buffer that a replayed step on the
+  // batch stream may still be using
+  DGK_HIP(hipStreamSynchronize(b->stream));
   for (dgk_solver *s : b->groups) DGK_HIP(hipStreamSynchronize(s->stream));
   for (size_t g = 0; g < b->groups.size(); ++g) {
     dgk_solver *s = b->groups[g];

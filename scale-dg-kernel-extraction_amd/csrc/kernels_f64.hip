@@ -31,6 +31,7 @@
 // stage_f64_lift<NQ> (stored Lift): the same arithmetic with Lift_mat read as
 // stored; used when Lift_mat is not of tensor-product form.
 #include <algorithm>
+#include <atomic>
 
 #include "dgk_internal.hpp"
 
@@ -466,14 +467,15 @@ stage_f64_lift(const StageArgs P) {
 
 template <int NQ, bool VREG, bool AXAL>
 int prepare_shape_valu() {
-  static int prepared_dev = -1;
+  // set once per device; concurrent callers may both set it (idempotent)
+  static std::atomic<int> prepared_dev{-1};
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
-  if (prepared_dev != dev) {
+  if (prepared_dev.load(std::memory_order_acquire) != dev) {
     DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG, AXAL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)Staged<NQ, VREG>::LDS_BYTES));
-    prepared_dev = dev;
+    prepared_dev.store(dev, std::memory_order_release);
   }
   return DGK_OK;
 }
@@ -663,12 +665,16 @@ constexpr int kValuShapes = 2;
 // staged VALU kernel else.
 constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
                                          0, 0, 0, 0};
-// Selected FP64 tile shape per Nq (see stage_f64_shapes).
-int g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3, 0, 0, 0, 0};
+// Selected FP64 tile shape per Nq (see stage_f64_shapes); atomic because a
+// serving worker may launch while another thread selects.
+std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1,
+                                            3, 3, 0, 0, 0, 0};
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
-  const int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0 : a.tile >= 0 ? a.tile : g_f64_shape[Nq];
+  const int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0
+                   : a.tile >= 0                ? a.tile
+                                                : g_f64_shape[Nq].load(std::memory_order_relaxed);
   if (Nq > kMaxNqValu && Nq <= kMaxNq) {
     if (a.flux_in || !a.ell)
       return set_error(DGK_ERR_UNSUPPORTED, "above p = 15 the stage runs on the MFMA kernel "
@@ -725,12 +731,12 @@ int stage_f64_select(int Nq, int cand) {
   if (cand == -1) cand = kF64Default[Nq];
   if (cand < 0 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
-  g_f64_shape[Nq] = cand;
+  g_f64_shape[Nq].store(cand, std::memory_order_relaxed);
   return DGK_OK;
 }
 
 int stage_f64_selected(int Nq) {
-  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_f64_shape[Nq];
+  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_f64_shape[Nq].load(std::memory_order_relaxed);
 }
 
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,

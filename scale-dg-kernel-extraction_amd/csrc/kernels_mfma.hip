@@ -36,6 +36,7 @@
 // orders; it adds the directions in the same order, so its results are the
 // same bits.
 #include <algorithm>
+#include <atomic>
 
 #include "dgk_internal.hpp"
 
@@ -524,14 +525,15 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
 
 template <class P, int NQ, int EPB, bool SEQ, bool AXAL>
 int prepare_form() {
-  static int prepared_dev = -1;
+  // set once per device; concurrent callers may both set it (idempotent)
+  static std::atomic<int> prepared_dev{-1};
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
-  if (prepared_dev != dev) {
+  if (prepared_dev.load(std::memory_order_acquire) != dev) {
     DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ, AXAL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)MP<P, NQ, EPB, SEQ>::LDS_BYTES));
-    prepared_dev = dev;
+    prepared_dev.store(dev, std::memory_order_release);
   }
   return DGK_OK;
 }
@@ -582,7 +584,8 @@ struct Cand {
 // direction shape at p = 14.
 constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 1, 1, 1, 1, 0, 0, 0, 0, 0,
                                         0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
-int g_mp_shape[kMaxNq + 1] = {0, 0, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
+std::atomic<int> g_mp_shape[kMaxNq + 1] = {0, 0, 1, 1, 1, 1, 0, 0, 0, 0, 0,
+                                           0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
 
 template <class P, int NQ>
 int prepare_nq_all() {
@@ -639,7 +642,8 @@ int prepare_stage_mp(int Nq) {
 
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
-#define DGK_CALL(N) launch_nq_cand<PrecMixed, N>(a.tile >= 0 ? a.tile : g_mp_shape[N], a, s)
+#define DGK_CALL(N) \
+  launch_nq_cand<PrecMixed, N>(a.tile >= 0 ? a.tile : g_mp_shape[N].load(std::memory_order_relaxed), a, s)
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
 }
@@ -678,12 +682,12 @@ int stage_mp_select(int Nq, int cand) {
   if (cand == -1) cand = kMpDefault[Nq];
   if (cand < 0 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
-  g_mp_shape[Nq] = cand;
+  g_mp_shape[Nq].store(cand, std::memory_order_relaxed);
   return DGK_OK;
 }
 
 int stage_mp_selected(int Nq) {
-  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_mp_shape[Nq];
+  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_mp_shape[Nq].load(std::memory_order_relaxed);
 }
 
 #undef DGK_NQ_SWITCH

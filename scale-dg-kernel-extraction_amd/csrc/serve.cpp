@@ -315,6 +315,9 @@ int start(dgk_server *sv, Req *r, int64_t off) {
   s.D = op.D; s.Lift = op.Lift; s.ell = op.use_ell ? op.ell : nullptr;
   s.D32 = op.D32; s.ell32 = op.ell32; s.x1D = op.x1D;
   s.axis_aligned = 1;
+  // the tile shape is fixed at admission, so a later dgk_tile_select from
+  // another thread never changes a request in flight
+  s.tile = mp ? stage_mp_selected(r->Nq) : stage_f64_selected(r->Nq);
   s.alternate = alternate_launches();
   s.launches = 0;
   // the z-plane-interleaved order of boxes with a multiple of 8 planes
@@ -495,7 +498,16 @@ int throttle(dgk_server *sv) {
     DGK_HIP(hipEventSynchronize(e));
     sv->event_pool.push_back(e);
   }
-  for (Req *r : t.finishing) DGK_TRY(retire(sv, r));
+  for (size_t k = 0; k < t.finishing.size(); ++k) {
+    const int rc = retire(sv, t.finishing[k]);
+    if (rc != DGK_OK) {
+      // the failed request and those after it have no result yet: leave them
+      // where fail_all posts one for each
+      sv->active.insert(sv->active.end(), t.finishing.begin() + (std::ptrdiff_t)k,
+                        t.finishing.end());
+      return rc;
+    }
+  }
   return DGK_OK;
 }
 
