@@ -74,6 +74,12 @@ def parse():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (the reference "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--stream-calibration", action="store_true",
+                    help="only run the streaming calibration kernel (five arrays read, "
+                         "one written) and print its rates")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only build the 512-instance ensemble in --form and run --steps "
+                         "graph-replayed steps (for kernel traces of a batched step)")
     return ap.parse_args()
 
 
@@ -120,6 +126,19 @@ def copy_rate(nbytes=1 << 30, reps=10):
         hip.hipFree(a)
         hip.hipFree(b)
     return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
+def stream_rate(dgk, n=1 << 24, reps=20):
+    """dgk_calibrate_stream: a kernel reading five arrays of n doubles and
+    writing one (the stage kernels' read:write mix), with 16-byte loads into
+    LDS and into registers; GB/s of bytes read plus written."""
+    import ctypes as C
+
+    a, r = C.c_double(0.0), C.c_double(0.0)
+    rc = dgk.lib().dgk_calibrate_stream(n, reps, C.byref(a), C.byref(r))
+    if rc:
+        raise RuntimeError(dgk.lib().dgk_last_error().decode())
+    return a.value, r.value
 
 
 def traffic_from_profile(kernel):
@@ -244,6 +263,27 @@ def batch_leg(dgk, inputs):
     out["speedup_batched_mp_vs_plain_fp64_p7"] = (
         out["batched_HIP_MP_p7"]["DOF-updates/s"] / out["plain_fp64_solo_p7"]["DOF-updates/s"])
     return out
+
+
+def batch_only(dgk, form, steps):
+    """The 512-instance ensemble in one form: autotune, capture, then `steps`
+    graph-replayed steps, nothing else (the command kernel traces wrap)."""
+    from dgk import inputs
+
+    ens = inputs.ensemble()
+    with dgk.Batch(form, ens) as b:
+        for k, it in enumerate(ens):
+            b.init_gaussian(k, it["centre"], it["sigma"], it["vel"])
+        tiles = b.autotune(reps=10)
+        b.step(1)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.synchronize()
+        t1 = time.perf_counter()
+    dof = sum((it["p"] + 1) ** 3 * it["ne"][0] ** 3 for it in ens)
+    print(json.dumps({"form": form, "steps": steps, "ms_per_step": 1e3 * (t1 - t0) / steps,
+                      "DOF-updates/s": dof * steps / (t1 - t0), "tiles": tiles}), flush=True)
 
 
 def flops_per_dof_update(p):
@@ -386,6 +426,13 @@ def main():
                                    "described in DESIGN.md, not provided as a run mode"}))
         sys.exit(2)
     import dgk
+    if args.stream_calibration:
+        a, r = stream_rate(dgk)
+        print(json.dumps({"stream_GBs": a, "stream_regs_GBs": r}), flush=True)
+        return
+    if args.batch_only:
+        batch_only(dgk, args.form, args.steps)
+        return
     if args.per_order_only:
         per_order_only(dgk, args.form, args.per_order_only, args.steps)
         return
@@ -426,6 +473,7 @@ def main():
     # traffic's rate, for reading `frac`
     roof = line["roofline"]
     roof["copy_GBs"] = copy_rate()
+    roof["stream_GBs"], roof["stream_regs_GBs"] = stream_rate(dgk)
     if roof["traffic"]:
         roof["traffic_GBs"] = roof["traffic"] / r["launch_s"] / 1e9
     if not args.no_extra:

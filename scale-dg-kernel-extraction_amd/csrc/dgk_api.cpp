@@ -152,6 +152,7 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
   a.dt_geo = 0;
   a.tile = s->tile;
+  a.geo_per_run = s->geo_per_run;
   return a;
 }
 
@@ -171,6 +172,7 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
     a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
     a.dt_geo = dt_geo;
     a.tile = s->tile;
+    a.geo_per_run = s->geo_per_run;
     return launch_stage_mp(s->Nq, a, stream);
   }
   StageArgs a = stage_args(s, qin, q0, out, mode, ca, cb, dt);
@@ -288,6 +290,14 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   // forwards).  The arithmetic of an element does not depend on when it runs.
   s->alternate = alternate_launches();
   s->zplane = t.zplane;
+  if (!t.geoid.empty()) {
+    const int T = run_elements(t.Nq);  // the mixed run kernel's runs are the same
+    for (int64_t e = 0; T > 0 && e < t.Ne; ++e)
+      if (t.geoid[e] != t.geoid[e - e % T]) {
+        s->geo_per_run = 0;
+        break;
+      }
+  }
   for (int b = 0; b < 3; ++b)
     if (hipMemsetAsync(s->q[b], 0, sizeof(double) * s->nfield, s->stream) != hipSuccess)
       return fail(set_error(DGK_ERR_DEVICE, "hipMemsetAsync failed"));

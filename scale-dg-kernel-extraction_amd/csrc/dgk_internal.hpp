@@ -223,6 +223,8 @@ struct StageArgs {
   int axis_aligned;       // face normals along the face axes (see MeshTables)
   int dt_geo;             // RK time step from the geometry record (per instance)
   int tile;               // tile-shape candidate, -1: the process-wide selection
+  int geo_per_run = 0;    // with geoid: every run of run_elements(Nq) elements from a
+                          // multiple of it shares one geometry record
 };
 
 // Mixed-precision operands (FP32 copies of the constant inputs).
@@ -246,18 +248,20 @@ struct StageArgsMP {
   int axis_aligned;
   int dt_geo;
   int tile;               // tile-shape candidate, -1: the process-wide selection
+  int geo_per_run = 0;    // as StageArgs::geo_per_run, for the mixed run kernel
 };
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
 // Tile shapes (elements per block) of the mixed-precision stage kernel of
-// order Nq: fills epb[0..2] and returns 3.  stage_mp_select picks the
+// order Nq (the MFMA shapes, then at p <= 4 the run kernel): fills epb and
+// returns their number (n >= 5).  stage_mp_select picks the
 // candidate that later launches of that Nq use.
 int stage_mp_shapes(int Nq, int *epb, int n);
 int stage_mp_select(int Nq, int cand);
 int stage_mp_selected(int Nq);
 // FP64: candidates 0, 1 are the VALU kernels, 2.. the FP64 MFMA kernel's shapes.
-constexpr int kMaxTileShapes = 6;
+constexpr int kMaxTileShapes = 8;
 int stage_f64_shapes(int Nq, int *epb, int n);
 int stage_f64_select(int Nq, int cand);
 int stage_f64_selected(int Nq);
@@ -266,6 +270,13 @@ int stage_f64_selected(int Nq);
 int tile_f64_shapes(int Nq, int *epb, int n);
 int prepare_tile_f64(int Nq);
 int launch_tile_f64(int Nq, int cand, const StageArgs &a, hipStream_t s);
+// FP64 stage kernel over runs of consecutive elements (kernels_run.hip):
+// the last FP64 candidate at p <= 7.  run_elements gives the elements per
+// workgroup (0: no run kernel for this Nq); launches it cannot take
+// (run_applicable false) run the VALU kernel instead, with the same bits.
+int run_elements(int Nq);
+bool run_applicable(int Nq, const StageArgs &a);
+int launch_run_f64(int Nq, const StageArgs &a, hipStream_t s);
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,
                             const double *v, const double *w,
                             const int32_t *nbr, const int32_t *vmapP,

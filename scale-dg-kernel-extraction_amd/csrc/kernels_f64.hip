@@ -672,7 +672,7 @@ std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
-  const int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0
+  int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0
                    : a.tile >= 0                ? a.tile
                                                 : g_f64_shape[Nq].load(std::memory_order_relaxed);
   if (Nq > kMaxNqValu && Nq <= kMaxNq) {
@@ -680,6 +680,12 @@ int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
       return set_error(DGK_ERR_UNSUPPORTED, "above p = 15 the stage runs on the MFMA kernel "
                                             "only: no given flux, separable Lift only");
     return launch_tile_f64(Nq, cand, a, s);
+  }
+  int mfma[kMaxTileShapes];
+  const int nm = tile_f64_shapes(Nq, mfma, kMaxTileShapes);
+  if (cand == kValuShapes + nm && run_elements(Nq) > 0) {
+    if (run_applicable(Nq, a)) return launch_run_f64(Nq, a, s);
+    cand = 1;  // the register VALU kernel: the same arithmetic
   }
   if (cand >= kValuShapes && !a.flux_in && a.ell)
     return launch_tile_f64(Nq, cand - kValuShapes, a, s);
@@ -721,7 +727,12 @@ int stage_f64_shapes(int Nq, int *epb, int n) {
 #undef DGK_F64_SHAPE
   default: return 0;
   }
-  return kValuShapes + tile_f64_shapes(Nq, epb + kValuShapes, n - kValuShapes);
+  const int nc = kValuShapes + tile_f64_shapes(Nq, epb + kValuShapes, n - kValuShapes);
+  if (run_elements(Nq) > 0 && nc < n) {
+    epb[nc] = run_elements(Nq);
+    return nc + 1;
+  }
+  return nc;
 }
 
 int stage_f64_select(int Nq, int cand) {

@@ -523,6 +523,334 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   }
 }
 
+// ---- mixed precision over runs of consecutive elements (p <= 4) -----------
+// stage_run_mp<NQ, AXAL>: the data flow of the FP64 run kernel
+// (kernels_run.hip) with the mixed form's arithmetic.  A workgroup advances
+// T consecutive elements: their q (FP64) and FP32 u, v, w blocks are one
+// contiguous range per field, copied to LDS 16 bytes per thread; faces whose
+// neighbour is one of the tile's elements take its traces from LDS, the
+// others gather them.  The A operands (FP64 volume fluxes E_d q u_d and the
+// FP64 face fluxes, each rounded once to FP32), the B operand, the MFMA tile
+// GEMMs and the FP64 combination and RK update are those of stage_mfma, so
+// every shape of the mixed form gives the same bits.
+typedef double dbl2_mp __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int face_axis_mp(int f) { return f >= 4 ? 2 : ((f & 1) ? 0 : 1); }
+
+template <int NQ>
+struct RunMP {
+  static constexpr int NQ2 = NQ * NQ, NP = NQ2 * NQ, NFT = 6 * NQ2;
+  static constexpr int T = NQ == 2 ? 32 : NQ == 3 ? 16 : NQ == 4 ? 8 : 4;
+  static constexpr int TNP = T * NP;
+  static_assert(TNP % 4 == 0, "16-byte aligned FP32 tiles");
+  static constexpr int PAIRS = TNP / 2;
+  static constexpr int THREADS = ((PAIRS + 63) / 64) * 64;
+  static constexpr int WAVES = THREADS / 64;
+  static constexpr int QF = 2 * THREADS;                    // LDS doubles of q
+  static constexpr int VCH = (TNP / 4 + 63) / 64;           // 1 KiB chunks of an FP32 field
+  static constexpr int VF = 256 * VCH;                      // LDS floats of u, v or w
+  static constexpr int ITEMS = T * NFT;
+  static constexpr int R = (ITEMS + THREADS - 1) / THREADS;
+  static constexpr int SLAB = R * THREADS;
+  static constexpr int NBR_REGS = (6 * T + 63) / 64;
+  static constexpr int ROWS = ((T * NQ2 + 15) / 16) * 16;  // pencils per direction
+  static constexpr int MTILES = ROWS / 16;
+  static constexpr int KSTEPS = (NQ + 2 + 3) / 4;
+  static constexpr int KPAD = 4 * KSTEPS;
+  static constexpr int AREA = ROWS * KPAD;
+  static constexpr int JOBS = 3 * MTILES;
+  static constexpr int JPW = (JOBS + WAVES - 1) / WAVES;
+};
+
+template <int NQ, bool AXAL>
+__global__ void __launch_bounds__(RunMP<NQ>::THREADS)
+stage_run_mp(const double *__restrict__ q_, const double *q0_, const float *__restrict__ u_,
+             const float *__restrict__ v_, const float *__restrict__ w_, double *out_,
+             const int32_t *__restrict__ nbr_, const float *__restrict__ D_,
+             const float *__restrict__ ell_, const double *__restrict__ geo_,
+             const uint32_t *__restrict__ geoid_, const int64_t Ne_, const double ca_,
+             const double cb_, const double dt_, const int mode_, const int dt_geo_,
+             const int rev_, const int64_t zpt_) {
+  using C = RunMP<NQ>;
+  using P = PrecMixed;
+  constexpr int NQ2 = C::NQ2, NP = C::NP, T = C::T, NFT = C::NFT, R = C::R;
+  constexpr int KPAD = C::KPAD, KSTEPS = C::KSTEPS, AREA = C::AREA;
+  constexpr int NTV = AXAL ? 1 : 3;  // traced FP32 velocity fields
+  __shared__ double sdbl[C::QF + C::SLAB + 24];
+  __shared__ float sflt[3 * C::VF + NTV * C::SLAB + 3 * AREA];
+  double *sq = sdbl, *tq = sq + C::QF, *sgeo = tq + C::SLAB;
+  float *su = sflt, *sv = su + C::VF, *sw = sv + C::VF;
+  float *tv = sw + C::VF;              // traced velocities, NTV slabs
+  float *As = tv + NTV * C::SLAB;      // A operands of the three directions
+  const float *svel[3] = {su, sv, sw};
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t ntile = (Ne_ + T - 1) / T;
+  const int64_t tile =
+      slot_element(xcd_block_mp(blockIdx.x, gridDim.x), ntile, nullptr, rev_, zpt_);
+  const int64_t e0 = tile * T;
+  const int nel = (int)min((int64_t)T, Ne_ - e0);
+  const int rem = nel * NP;
+  const double *g = geo_ + (geoid_ ? (int64_t)geoid_[e0] * kGeoStride : 0);
+
+  int32_t nbv[C::NBR_REGS];
+#pragma unroll
+  for (int c = 0; c < C::NBR_REGS; ++c) {
+    const int x = c * 64 + lane;
+    nbv[c] = x < 6 * nel ? nbr_[e0 * 6 + x] : -1;
+  }
+
+  // ---- own fields: q 16 bytes (two nodes) and u, v, w 16 bytes (four nodes)
+  // per thread, async into LDS ------------------------------------------------
+  const bool qfull = 2 * tid + 1 < rem, qhalf = 2 * tid + 1 == rem;
+  const bool vfull = 4 * tid + 3 < rem;
+  const int vpart = vfull ? 0 : max(0, rem - 4 * tid);  // 1..3 nodes of a last granule
+  const int64_t own = e0 * NP + 2 * tid, own4 = e0 * NP + 4 * tid;
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (qfull)
+    __builtin_amdgcn_global_load_lds((const void *)(q_ + own),
+                                     (__attribute__((address_space(3))) void *)(sq + wave * 128),
+                                     16, 0, 0);
+  if (vfull) {
+    const float *src[3] = {u_, v_, w_};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(src[d] + own4),
+          (__attribute__((address_space(3))) void *)((float *)svel[d] + wave * 256), 16, 0, 0);
+  }
+#endif
+  double qtail = 0.0;
+  if (qhalf) qtail = q_[own];
+  float vtail[3][3] = {};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c < vpart) {
+      vtail[0][c] = u_[own4 + c];
+      vtail[1][c] = v_[own4 + c];
+      vtail[2][c] = w_[own4 + c];
+    }
+  }
+
+  // ---- neighbour traces of faces that leave the tile -----------------------
+  int32_t nb_r[R];
+  double tq_r[R];
+  float tv_r[R][NTV];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    tq_r[r] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NTV; ++c) tv_r[r][c] = 0.f;
+    const int it = tid + r * C::THREADS;
+    const int el = it / NFT, f = (it - el * NFT) / NQ2, ab = it - el * NFT - f * NQ2;
+    const int x = el * 6 + f;
+    int32_t nb = -1;
+#pragma unroll
+    for (int c = 0; c < C::NBR_REGS; ++c) {
+      const int32_t sv_ = __shfl(nbv[c], x & 63);
+      if ((x >> 6) == c) nb = sv_;
+    }
+    nb_r[r] = el < nel ? nb : -1;
+    if (el < nel && (nb < e0 || nb >= e0 + nel)) {
+      const int64_t pidx = (int64_t)nb * NP + nbr_face_node(f, ab % NQ, ab / NQ, NQ);
+      tq_r[r] = q_[pidx];
+      if constexpr (AXAL) {
+        const int ax = face_axis_mp(f);
+        tv_r[r][0] = (ax == 0 ? u_ : ax == 1 ? v_ : w_)[pidx];
+      } else {
+        tv_r[r][0] = u_[pidx];
+        tv_r[r][1] = v_[pidx];
+        tv_r[r][2] = w_[pidx];
+      }
+    }
+  }
+
+  // ---- RK base, B operand, geometry ------------------------------------------
+  const bool rk = mode_ == 1;
+  const bool use_q0 = rk && ca_ != 0.0;
+  dbl2_mp q0v = {0.0, 0.0};
+  if (use_q0) {
+    if (qfull) q0v = __builtin_nontemporal_load((const dbl2_mp *)(q0_ + own));
+    else if (qhalf) q0v.x = __builtin_nontemporal_load(q0_ + own);
+  }
+  const int col = lane & 15, kq = lane >> 4;
+  float bop[1][KSTEPS];
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int l = kq * KSTEPS + ks;
+    bop[0][ks] = col >= NQ ? 0.f : l < NQ ? D_[col + l * NQ] : l == NQ ? ell_[col]
+                 : l == NQ + 1 ? ell_[NQ + col] : 0.f;
+  }
+  static_assert(kGeoNormal == kGeoFscale + 6, "Fscale and normals are adjacent");
+  if (tid < 24) sgeo[tid] = g[kGeoFscale + tid];
+  __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
+
+  if (qhalf) sq[2 * tid] = qtail;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c < vpart) {
+      su[4 * tid + c] = vtail[0][c];
+      sv[4 * tid + c] = vtail[1][c];
+      sw[4 * tid + c] = vtail[2][c];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int it = tid + r * C::THREADS;
+    tq[it] = tq_r[r];
+#pragma unroll
+    for (int c = 0; c < NTV; ++c) tv[c * C::SLAB + it] = tv_r[r][c];
+  }
+  __syncthreads();
+
+  // ---- FP64 upwind fluxes, rounded into the Lift columns of A ---------------
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int it = tid + r * C::THREADS;
+    const int el = it / NFT, f = (it - el * NFT) / NQ2, ab = it - el * NFT - f * NQ2;
+    const int32_t nb = nb_r[r];
+    if (nb >= 0) {
+      const int a = ab % NQ, b = ab / NQ;
+      const bool inside = nb >= e0 && nb < e0 + nel;
+      const int m = el * NP + own_face_node(f, a, b, NQ);
+      const int pm = inside ? (int)(nb - e0) * NP + nbr_face_node(f, a, b, NQ) : 0;
+      const double qM = sq[m];
+      const double qP = inside ? sq[pm] : tq[it];
+      const int ax = face_axis_mp(f);
+      const double n1 = sgeo[6 + 3 * f], n2 = sgeo[7 + 3 * f], n3 = sgeo[8 + 3 * f];
+      double VelM, VelP;
+      if constexpr (AXAL) {
+        const float *sf = ax == 0 ? su : ax == 1 ? sv : sw;
+        const double na = ax == 0 ? n1 : ax == 1 ? n2 : n3;
+        VelM = (double)sf[m] * na;
+        VelP = (double)(inside ? sf[pm] : tv[it]) * na;
+      } else {
+        VelM = (double)su[m] * n1 + (double)sv[m] * n2 + (double)sw[m] * n3;
+        const float up = inside ? su[pm] : tv[it];
+        const float vp = inside ? sv[pm] : tv[C::SLAB + it];
+        const float wp = inside ? sw[pm] : tv[2 * C::SLAB + it];
+        VelP = (double)up * n1 + (double)vp * n2 + (double)wp * n3;
+      }
+      const double alpha = 0.5 * fabs(VelP + VelM);
+      const float fl = (float)(0.5 * sgeo[f] * (qP * VelP - qM * VelM - alpha * (qP - qM)));
+      // face f is the minus (faces 3, 0, 4) or plus (1, 2, 5) face of its axis
+      const bool plus = f == 1 || f == 2 || f == 5;
+      float *row = As + ax * AREA + (el * NQ2 + ab) * KPAD;
+      row[NQ + (plus ? 1 : 0)] = fl;
+      if (!plus) {
+#pragma unroll
+        for (int l = NQ + 2; l < KPAD; ++l) row[l] = 0.f;
+      }
+    }
+  }
+
+  // ---- FP64 volume fluxes E_d q u_d, rounded into A ---------------------------
+  const double E3[3] = {g[kGeoEscale], g[kGeoEscale + 1], g[kGeoEscale + 2]};
+  double qc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int n = 2 * tid + r;
+    if (n < rem) {
+      const int el = n / NP, nn = n - el * NP;
+      const int i = nn % NQ, j = (nn / NQ) % NQ, k = nn / NQ2, R0 = el * NQ2;
+      const double qq = sq[n];
+      qc[r] = qq;
+      As[0 * AREA + (R0 + j + k * NQ) * KPAD + i] = (float)(E3[0] * (qq * (double)su[n]));
+      As[1 * AREA + (R0 + i + k * NQ) * KPAD + j] = (float)(E3[1] * (qq * (double)sv[n]));
+      As[2 * AREA + (R0 + i + j * NQ) * KPAD + k] = (float)(E3[2] * (qq * (double)sw[n]));
+    }
+  }
+  __syncthreads();
+
+  // ---- MFMA GEMMs over (direction, 16-pencil tile) jobs ---------------------
+  const int rd = col * KPAD + kq * KSTEPS;
+  const int wr = P::RSTRIDE_Q * kq * KPAD + col;
+#pragma unroll
+  for (int jj = 0; jj < C::JPW; ++jj) {
+    const int job = wave + jj * C::WAVES;
+    if (job >= C::JOBS) break;  // wave-uniform
+    const int d = job / C::MTILES, t = job - d * C::MTILES;
+    mfma_tile<P, KSTEPS, KPAD, NQ, 1>(As + d * AREA + t * 16 * KPAD, bop, rd, wr, col);
+  }
+  __syncthreads();
+
+  // ---- FP64 sum of the directions, RK epilogue ---------------------------------
+  const double dtv = dt_geo_ ? g[kGeoDt] : dt_;
+  double res[2] = {0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int n = 2 * tid + r;
+    if (n < rem) {
+      const int el = n / NP, nn = n - el * NP;
+      const int i = nn % NQ, j = (nn / NQ) % NQ, k = nn / NQ2, R0 = el * NQ2;
+      const double tend = ((double)As[0 * AREA + (R0 + j + k * NQ) * KPAD + i] +
+                           (double)As[1 * AREA + (R0 + i + k * NQ) * KPAD + j]) +
+                          (double)As[2 * AREA + (R0 + i + j * NQ) * KPAD + k];
+      const double td = -tend;
+      if (!rk) {
+        res[r] = td;
+      } else {
+        const double upd = cb_ * (qc[r] + dtv * td);
+        res[r] = use_q0 ? ca_ * (r == 0 ? q0v.x : q0v.y) + upd : upd;
+      }
+    }
+  }
+  if (qfull) {
+    const dbl2_mp v = {res[0], res[1]};
+    __builtin_nontemporal_store(v, (dbl2_mp *)(out_ + own));
+  } else if (qhalf) {
+    __builtin_nontemporal_store(res[0], out_ + own);
+  }
+}
+
+template <int NQ, bool AXAL>
+int launch_run_mp_nq(const StageArgsMP &a, hipStream_t s) {
+  using C = RunMP<NQ>;
+  const int64_t ntile = (a.Ne + C::T - 1) / C::T;
+  if (ntile > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
+  int64_t zpt = 0;
+  if (a.zplane > 0 && a.zplane % C::T == 0) {
+    const int64_t P = a.zplane / C::T;
+    if (ntile % (8 * P) == 0) zpt = P;
+  }
+  hipLaunchKernelGGL((stage_run_mp<NQ, AXAL>), dim3((unsigned)ntile), dim3(C::THREADS), 0, s,
+                     a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr, a.D, a.ell, a.geo, a.geoid, a.Ne,
+                     a.ca, a.cb, a.dt, a.mode, a.dt_geo, a.reverse, zpt);
+  DGK_HIP(hipGetLastError());
+  return DGK_OK;
+}
+
+// the FP64 run kernel's elements per workgroup at the same orders (the
+// solver's per-run geometry check covers both)
+constexpr int run_mp_elements(int NQ) {
+  return NQ == 2 ? 32 : NQ == 3 ? 16 : NQ == 4 ? 8 : NQ == 5 ? 4 : 0;
+}
+
+bool aligned16_mp(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+// The mixed run kernel takes the launch (true), or the launch needs another
+// kernel (false): node connectivity, an order array, per-element geometry
+// within a tile, or unaligned fields.
+bool run_mp_applicable(int Nq, const StageArgsMP &a) {
+  const int T = run_mp_elements(Nq);
+  if (T < 1 || !a.nbr || a.vmapP || a.order) return false;
+  if (a.geoid && !a.geo_per_run) return false;
+  const bool q0_ok = a.mode != 1 || a.ca == 0.0 || aligned16_mp(a.q0);
+  return aligned16_mp(a.q) && aligned16_mp(a.u) && aligned16_mp(a.v) && aligned16_mp(a.w) &&
+         aligned16_mp(a.out) && q0_ok;
+}
+
+int launch_run_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
+  const bool ax = a.axis_aligned != 0;
+  switch (Nq) {
+  case 2: return ax ? launch_run_mp_nq<2, true>(a, s) : launch_run_mp_nq<2, false>(a, s);
+  case 3: return ax ? launch_run_mp_nq<3, true>(a, s) : launch_run_mp_nq<3, false>(a, s);
+  case 4: return ax ? launch_run_mp_nq<4, true>(a, s) : launch_run_mp_nq<4, false>(a, s);
+  case 5: return ax ? launch_run_mp_nq<5, true>(a, s) : launch_run_mp_nq<5, false>(a, s);
+  default: return set_error(DGK_ERR_UNSUPPORTED, "mixed run kernel: Nq outside 2..5");
+  }
+}
+
 template <class P, int NQ, int EPB, bool SEQ, bool AXAL>
 int prepare_form() {
   // set once per device; concurrent callers may both set it (idempotent)
@@ -642,8 +970,16 @@ int prepare_stage_mp(int Nq) {
 
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
-#define DGK_CALL(N) \
-  launch_nq_cand<PrecMixed, N>(a.tile >= 0 ? a.tile : g_mp_shape[N].load(std::memory_order_relaxed), a, s)
+  int cand = a.tile;
+  if (cand < 0 && Nq >= kMinNq && Nq <= kMaxNq) cand = g_mp_shape[Nq].load(std::memory_order_relaxed);
+  // the last candidate at p <= 4 is the run kernel; launches it cannot take
+  // run the first shape, with the same bits
+  int e[kMaxTileShapes];
+  if (run_mp_elements(Nq) > 0 && cand == stage_mp_shapes(Nq, e, kMaxTileShapes) - 1) {
+    if (run_mp_applicable(Nq, a)) return launch_run_mp(Nq, a, s);
+    cand = 0;
+  }
+#define DGK_CALL(N) launch_nq_cand<PrecMixed, N>(cand, a, s)
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
 }
@@ -661,11 +997,20 @@ int launch_tile_f64(int Nq, int cand, const StageArgs &a, hipStream_t s) {
 #undef DGK_CALL
 }
 
-int stage_mp_shapes(int Nq, int *epb, int n) {
-  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < kKinds) return 0;
+static int mp_mfma_shapes(int Nq, int *epb) {
 #define DGK_CALL(N) shapes_of<PrecMixed, N>(epb)
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
+}
+
+int stage_mp_shapes(int Nq, int *epb, int n) {
+  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < kKinds + 1) return 0;
+  const int nc = mp_mfma_shapes(Nq, epb);
+  if (nc > 0 && run_mp_elements(Nq) > 0) {
+    epb[nc] = run_mp_elements(Nq);
+    return nc + 1;
+  }
+  return nc;
 }
 
 int tile_f64_shapes(int Nq, int *epb, int n) {
@@ -676,8 +1021,8 @@ int tile_f64_shapes(int Nq, int *epb, int n) {
 }
 
 int stage_mp_select(int Nq, int cand) {
-  int e[kKinds];
-  const int nc = stage_mp_shapes(Nq, e, kKinds);
+  int e[kMaxTileShapes];
+  const int nc = stage_mp_shapes(Nq, e, kMaxTileShapes);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
   if (cand == -1) cand = kMpDefault[Nq];
   if (cand < 0 || cand >= nc)

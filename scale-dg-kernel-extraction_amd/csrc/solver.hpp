@@ -41,6 +41,7 @@ struct dgk_solver {
   uint64_t launches = 0;     // stage launches so far (picks the direction)
   int axis_aligned = 0;
   int tile = -1;             // tile-shape candidate chosen for this mesh (-1: process default)
+  int geo_per_run = 1;       // every run of run_elements(Nq) elements shares a geometry record
   double *x1D = nullptr;     // LGL nodes (device), for device-side initial conditions
   BoxShape box;              // structured box (NeX == 0 when built from arrays)
   int32_t *nbr = nullptr, *vmapP = nullptr, *halo = nullptr;

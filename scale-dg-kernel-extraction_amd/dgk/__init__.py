@@ -59,6 +59,7 @@ _SIGNATURES = {
     "dgk_rk_update": (_I, [_P, _I, _D]),
     "dgk_step": (_I, [_P, _D, _I]),
     "dgk_minmax": (_I, [_P, _P, _P]),
+    "dgk_calibrate_stream": (_I, [_L, _I, _P, _P]),
     "dgk_synchronize": (_I, [_P]),
     "dgk_set_kernel_timing": (_I, [_P, _I]),
     "dgk_timer_report": (_I, [_P, _P, _P]),
@@ -150,8 +151,8 @@ def tile_shapes(evaltype, p: int) -> list:
     """Elements per workgroup of each tile-shape candidate of a stage kernel."""
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
-    e, n = np.zeros(6, np.int32), C.c_int(0)
-    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 6, C.byref(n)))
+    e, n = np.zeros(8, np.int32), C.c_int(0)
+    _check(lib().dgk_tile_shapes(int(evaltype), int(p), _ptr(e), 8, C.byref(n)))
     return e[:n.value].tolist()
 
 
@@ -167,7 +168,15 @@ def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True) -> str:
         b = lambda x: "true" if x else "false"
         return f"stage_f64<{nq}, {b(cand == 1)}, {b(axis_aligned)}>"
     shapes = tile_shapes(evaltype, p)
+    # the last candidate is the kernel over runs of elements: FP64 at p <= 7,
+    # mixed at p <= 4
+    has_run = nq <= 8 if evaltype == EVAL_HIP else nq <= 5
+    if has_run and cand == len(shapes) - 1:
+        name = "stage_run" if evaltype == EVAL_HIP else "stage_run_mp"
+        return f"{name}<{nq}, {'true' if axis_aligned else 'false'}>"
     mfma = shapes[nvalu:]
+    if has_run:
+        mfma = mfma[:-1]
     mc = cand - nvalu
     epb = mfma[mc]
     seq = nq > 16 or (mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or epb == 1))

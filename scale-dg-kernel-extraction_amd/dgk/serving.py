@@ -161,21 +161,26 @@ def _pct(x, q):
 
 
 def run_stream(n: int = 200, rate: float = 4.0, seed: int = 5, hbm_budget: int = 64 << 30,
-               max_active: int = 64, nstreams: int = 4, sample: int = 8,
-               sample_max_dof: int = 4_000_000, progress: bool = True,
-               policy: int = TIME_FAIR, max_steps_per_tick: int = 64) -> dict:
+               max_active: int = 64, nstreams: int = 4, sample: int | None = None,
+               sample_max_dof: int | None = None, progress: bool = True,
+               policy: int = TIME_FAIR, max_steps_per_tick: int = 64,
+               return_fields: bool = False) -> dict:
     """Replay the first n requests of the seeded Poisson stream in real time.
 
-    Up to `sample` requests of each tolerance class with at most
-    sample_max_dof DOF return their final field; after the stream each of them
-    is solved again alone (dgk_solver) in its own form and in plain FP64, to
-    show that the served result equals the solve alone and to state its error
-    against the FP64 result."""
+    Every request (sample=None), or up to `sample` requests of each tolerance
+    class with at most sample_max_dof DOF, returns its final field; after the
+    stream each of them is solved again alone (dgk_solver) in its own form and,
+    for the 1e-5 class, in plain FP64, to show that the served result equals
+    the solve alone and to state its error against the FP64 result.  With
+    return_fields the served fields and the FP64 solves are returned too
+    (out["fields"][k] = (served, fp64))."""
     reqs = inputs.serve_requests(n=n, seed=seed, rate=rate)
     picked = {TOL_FP64: [], TOL_1E5: []}
     for k, r in enumerate(reqs):
         dof = int(np.prod(r["ne"])) * (r["p"] + 1) ** 3
-        if dof <= sample_max_dof and len(picked[r["tol_class"]]) < sample:
+        if sample_max_dof is not None and dof > sample_max_dof:
+            continue
+        if sample is None or len(picked[r["tol_class"]]) < sample:
             picked[r["tol_class"]].append(k)
     wanted = set(picked[TOL_FP64] + picked[TOL_1E5])
     results = {}
@@ -230,12 +235,14 @@ def run_stream(n: int = 200, rate: float = 4.0, seed: int = 5, hbm_budget: int =
         out["by_class"][name] = {"count": len(sel), "p50_latency_s": _pct(sel, 50),
                                  "p99_latency_s": _pct(sel, 99)}
     # served results against the same solves alone
-    errs = []
+    errs, fields = [], {}
     for k in sorted(wanted):
         d, r = results[k], reqs[k]
         alone = solve_alone(r, d["form"], d["dt"])
         ref = alone if d["form"] == "HIP" else solve_alone(r, "HIP", d["dt"])
         q = d["q"]
+        if return_fields:
+            fields[k] = (q, ref)
         errs.append({
             "request": k, "form": d["form"], "dof": d["dof"], "nstep": d["nstep"],
             "same_as_alone": bool(np.array_equal(q, alone)),
@@ -249,4 +256,8 @@ def run_stream(n: int = 200, rate: float = 4.0, seed: int = 5, hbm_budget: int =
         "max_rel_fp64_class": max([e["max_rel_vs_fp64"] for e in errs
                                    if e["form"] == "HIP"], default=None),
         "requests": errs}
+    if return_fields:
+        out["fields"] = fields
+        out["reqs"] = reqs
+        out["results"] = done
     return out

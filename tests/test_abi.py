@@ -143,14 +143,19 @@ def test_serve_request_stream_ranges():
 
 def test_tile_shapes_host_side():
     # p <= 7: one, two, four wavefronts of elements, then one wavefront
-    # direction by direction
+    # direction by direction; at p <= 4 last the run of consecutive elements
     assert dgk.tile_shapes("HIP_MP", 7) == [1, 2, 4, 1]
-    assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16, 4]
+    assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16, 4, 8]
+    assert dgk.kernel_name("HIP_MP", 3, 4) == "stage_run_mp<4, true>"
     # p >= 8: one element, two elements, one element direction by direction
     assert dgk.tile_shapes("HIP_MP", 15) == [1, 2, 1]
     # FP64: the two VALU kernels, then the FP64 MFMA kernel's shapes (two
-    # elements of p = 15 would need more than 160 KiB of LDS)
-    assert dgk.tile_shapes("HIP", 7) == [1, 1, 1, 2, 4, 1]
+    # elements of p = 15 would need more than 160 KiB of LDS), at p <= 7 last
+    # the run kernel
+    assert dgk.tile_shapes("HIP", 7) == [1, 1, 1, 2, 4, 1, 1]
+    assert dgk.tile_shapes("HIP", 3) == [4, 4, 4, 8, 16, 4, 8]
+    assert dgk.kernel_name("HIP", 7, 6) == "stage_run<8, true>"
+    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true, false>"
     assert dgk.tile_shapes("HIP", 15) == [1, 1, 1, 1]
     with pytest.raises(dgk.DgkError) as ei:
         dgk.tile_select("HIP", 15, 4)

@@ -88,3 +88,34 @@ def test_trace_matches_reference_driver(tmp_path, opt, extra, tol):
     # (A30) truncates the last label to 30 characters, in both programs
     for name in ("Main:", "Cal_tend:", "Element boundary flux:", "Volume derivate + surface lift"):
         assert name in got.stdout and name in ref.stdout
+
+
+REF_DGK_HOST = ROOT / "oracle" / "_ref" / "ref_dgk_host"
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not REF_DGK_HOST.exists(), reason="reference-array host not built")
+@pytest.mark.parametrize("ne,p,dev,cpu,tol", [
+    ((4, 3, 5), 3, "HIP", "GENERAL", 1e-12),
+    ((3, 2, 4), 7, "HIP", "GENERAL", 1e-12),
+    ((2, 3, 2), 11, "HIP", "OPT1", 1e-12),
+    ((4, 3, 5), 3, "HIP_MP", "GENERAL", 1e-5),
+])
+def test_reference_mesh_arrays_through_dgk_create(ne, p, dev, cpu, tol):
+    """The arrays the reference's own mesh_setup builds (mod_mesh.f90:74-193)
+    go to dgk_create unchanged through mod_dgk (the INTEGRATION.md route);
+    the device loop's per-step min/max and final field match the reference
+    modules' CPU loop run in the same process on the same inputs."""
+    nstep, dt = 12, 2e-3 / (p + 1)
+    r = subprocess.run([str(REF_DGK_HOST), *map(str, ne), str(p), dev, cpu, str(nstep), repr(dt)],
+                       cwd=str(PKG), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    steps = re.findall(r"^STEP\s+(\d+)\s+(\S+)\s+(\S+)\s+(\S+)\s+(\S+)\s*$", r.stdout, re.M)
+    assert [int(s[0]) for s in steps] == list(range(1, nstep + 1))
+    rows = [[float(x) for x in s[1:]] for s in steps]
+    scale = max(max(abs(x) for x in row) for row in rows)
+    for dmin, dmax, cmin, cmax in rows:
+        assert abs(dmin - cmin) <= tol * scale and abs(dmax - cmax) <= tol * scale
+    maxrel = float(re.search(r"^MAXREL\s+(\S+)", r.stdout, re.M).group(1))
+    rell2 = float(re.search(r"^RELL2\s+(\S+)", r.stdout, re.M).group(1))
+    assert (maxrel if dev == "HIP" else rell2) <= tol, (maxrel, rell2)

@@ -185,10 +185,11 @@ def test_c2_full_size_short_run(nstep):
 @pytest.mark.parametrize("p", range(1, 16))
 def test_fp64_tile_shapes_against_oracle(p):
     """Every FP64 tile shape (candidates 0, 1: VALU kernels with v, w staged
-    or in registers; 2..: FP64 MFMA kernel shapes) holds the tendency and
-    loop bars against the oracle; the two VALU kernels give the same bits, and
-    so do the MFMA shapes among themselves, also with a partial last
-    workgroup (30 elements)."""
+    or in registers; then the FP64 MFMA kernel shapes; at p <= 7 last the
+    kernel over runs of consecutive elements) holds the tendency and loop
+    bars against the oracle; the two VALU kernels and the run kernel give the
+    same bits, and so do the MFMA shapes among themselves, also with a partial
+    last workgroup (30 elements)."""
     ne = (3, 2, 5) if p <= 7 else (2, 1, 3)
     m = oracle.Mesh(*ne, p)
     q, u, v, w = inputs.random_fields(*ne, p, seed=900 + p, vel_scale=0.8)
@@ -211,8 +212,49 @@ def test_fp64_tile_shapes_against_oracle(p):
     finally:
         dgk.tile_select("HIP", p, -1)
     assert np.array_equal(loops[1], loops[0])
-    for k in range(3, len(loops)):
+    nmfma = len(loops) - 2
+    if p <= 7:
+        assert dgk.kernel_name("HIP", p, len(loops) - 1).startswith("stage_run<")
+        assert np.array_equal(loops[-1], loops[0])
+        nmfma -= 1
+    for k in range(3, 2 + nmfma):
         assert np.array_equal(loops[k], loops[2]), k
+
+
+@pytest.mark.parametrize("p", range(1, 8))
+def test_run_kernel_uniform_tilted_normals(p):
+    """The run kernel's general face-normal form: one geometry record with
+    off-axis normals shared by every element (so the launch is not handed to
+    another kernel), against the oracle and with the VALU kernel's bits."""
+    ne = (4, 3, 2)
+    m = oracle.Mesh(*ne, p)
+    nf = m.normal_fn.copy()
+    Nfp = m.Nfp
+    for f in range(6):
+        seg = slice(f * Nfp, (f + 1) * Nfp)
+        a = int(np.argmax(np.abs(nf[:, 0, f * Nfp])))
+        nf[(a + 1) % 3, :, seg] = 0.3
+        nf[(a + 2) % 3, :, seg] = -0.2
+    m.normal_fn = nf
+    q, u, v, w = inputs.random_fields(*ne, p, seed=1500 + p, vel_scale=0.8)
+    dt = 0.2 * inputs.cfl_dt(inputs.lgl_nodes(p), ne, (0.8, 0.8, 0.8))
+    tend_ref = m.cal_tend(m.field(q), m.field(u), m.field(v), m.field(w))
+    loop_ref, _, _ = m.run(q, u, v, w, dt, 3)
+    run = len(dgk.tile_shapes("HIP", p)) - 1
+    got = {}
+    try:
+        for cand in (0, run):
+            dgk.tile_select("HIP", p, cand)
+            with solver_from_mesh(m) as s:
+                s.set_fields(q, u, v, w)
+                tend = s.cal_tend()
+                s.step(dt, 3)
+                got[cand] = (tend, s.get_field(dgk.FIELD_Q))
+    finally:
+        dgk.tile_select("HIP", p, -1)
+    assert maxrel(got[run][0], tend_ref) <= TEND_TOL
+    assert maxrel(got[run][1], loop_ref) <= LOOP_TOL
+    assert np.array_equal(got[run][0], got[0][0]) and np.array_equal(got[run][1], got[0][1])
 
 
 @pytest.mark.parametrize("form", ["HIP", "HIP_MP"])
