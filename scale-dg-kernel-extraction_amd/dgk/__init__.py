@@ -404,8 +404,8 @@ class Solver:
         """Time every tile shape of this solver's form and order on its mesh and
         keep the fastest for this solver."""
         best, n = C.c_int(0), C.c_int(0)
-        us = np.zeros(6)
-        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 6, C.byref(n)))
+        us = np.zeros(8)
+        _check(lib().dgk_autotune(self._h, int(reps), C.byref(best), _ptr(us), 8, C.byref(n)))
         return dict(best=best.value, us_per_launch=us[:n.value].tolist())
 
 
