@@ -180,6 +180,19 @@ int dgk_step(dgk_solver *s, double dt, int nstep);
 int dgk_minmax(dgk_solver *s, double *qmin, double *qmax);
 /* Wait for the solver's stream. */
 int dgk_synchronize(dgk_solver *s);
+/* min/max of n doubles at the device address q into result[0] (min) and
+ * result[1] (max), also a device address, enqueued on stream (a hipStream_t,
+ * NULL for the null stream) without waiting, allocating nothing: the partials
+ * of the reduction live in a 16 KiB static device array that each GPU holds
+ * until the process ends.  No counterpart in the reference; it is for callers
+ * that cannot allocate (code under HIP graph capture -- make the first call
+ * on each device outside a capture, it looks up the array's address -- and
+ * tight serving loops).  Any n from 1 to 2^40 fits the fixed partials;
+ * larger or smaller n, NULL pointers, host pointers and ranges that run past
+ * their device allocation are refused with DGK_ERR_ARG.  Calls that overlap
+ * on one GPU share the array and race; calls ordered on one stream do not.
+ * Same values as dgk_minmax of the same field. */
+int dgk_minmax_noalloc(const double *q, int64_t n, double *result, void *stream);
 
 /* ---- timers (mod_common.f90:33-71, names of main.f90:175-177 and
  * mod_advect3d_eq.f90:34-35) ---------------------------------------------

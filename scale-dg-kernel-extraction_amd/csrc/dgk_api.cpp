@@ -617,6 +617,40 @@ int dgk_minmax(dgk_solver *s, double *qmin, double *qmax) {
   return resolve_timers(s);  // the stream is idle: folding costs no wait
 }
 
+// Bounds of a device allocation holding [p, p + bytes), or an error: a host
+// pointer or a range past the allocation's end would fault the GPU.
+static int check_device_range(const void *p, int64_t bytes, const char *what) {
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(DGK_ERR_ARG, std::string(what) + " is not device memory");
+  }
+  if (at.type != hipMemoryTypeDevice)
+    return set_error(DGK_ERR_ARG, std::string(what) + " is not device memory");
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(DGK_ERR_ARG, std::string(what) + ": no device allocation holds it");
+  }
+  const int64_t off = (const char *)p - (const char *)base;
+  if (off < 0 || off + bytes > (int64_t)size)
+    return set_error(DGK_ERR_ARG, std::string(what) + ": " + std::to_string(bytes) +
+                                      " bytes run past its allocation");
+  return DGK_OK;
+}
+
+int dgk_minmax_noalloc(const double *q, int64_t n, double *result, void *stream) {
+  clear_error();
+  if (!q || !result) return set_error(DGK_ERR_ARG, "null argument");
+  if (n < 1 || n > kNoallocMaxN)
+    return set_error(DGK_ERR_ARG, "n = " + std::to_string(n) + " outside 1.." +
+                                      std::to_string(kNoallocMaxN));
+  DGK_TRY(check_device_range(q, n * (int64_t)sizeof(double), "q"));
+  DGK_TRY(check_device_range(result, 2 * (int64_t)sizeof(double), "result"));
+  return launch_minmax_noalloc(q, n, result, (hipStream_t)stream);
+}
+
 int dgk_synchronize(dgk_solver *s) {
   DGK_TRY(enter(s));
   DGK_HIP(hipStreamSynchronize(s->stream));

@@ -294,6 +294,11 @@ int launch_init_gaussian(double *q, double *u, double *v, double *w, const doubl
                          const double c[3], double sigma, const double vel[3],
                          hipStream_t s);
 constexpr int kMinmaxBlocks = 1024;
+// min/max of n doubles into result[0..1] with the partials in the library's
+// static device scratch (kNoallocBlocks pairs); allocates nothing.
+constexpr int kNoallocBlocks = 1024;
+constexpr int64_t kNoallocMaxN = (int64_t)1 << 40;  // 8 TiB of doubles
+int launch_minmax_noalloc(const double *q, int64_t n, double *result, hipStream_t s);
 
 // Periodic neighbour table of a NeX x NeY x NeZ box built on the device, with
 // element indices offset by elem_off.

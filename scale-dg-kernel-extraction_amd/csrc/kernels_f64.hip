@@ -32,6 +32,7 @@
 // stored; used when Lift_mat is not of tensor-product form.
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 
 #include "dgk_internal.hpp"
 
@@ -787,6 +788,48 @@ int launch_minmax(const double *q, int64_t n, double *partial, int nblk,
   hipLaunchKernelGGL(minmax_final, dim3(1), dim3(256), 0, s, partial, nblk, result);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
+}
+
+// ---- min/max without allocation (dgk_minmax_noalloc) ----------------------
+// The per-block partials live in `scratch`, a file-scope device array of
+// kNoallocBlocks (min, max) pairs = 16 KiB.  Every GPU the library runs on
+// holds its own copy until the process ends.  In host code the array's name
+// is not a device address: the address of each device's copy is looked up
+// once per device (hipGetSymbolAddress, with that device current) and only
+// that address is handed to kernels.  A call writes the pairs of the blocks
+// it launches before its final fold reads exactly those pairs, so nothing has
+// to start at zero; calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) double scratch[2 * kNoallocBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device
+}  // namespace
+
+static int scratch_address(double **out) {
+  int dev = 0;
+  DGK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(DGK_ERR_DEVICE, "device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    double *p = nullptr;
+    DGK_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(DGK_ERR_DEVICE, "hipGetSymbolAddress(scratch) gave NULL");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return DGK_OK;
+}
+
+int launch_minmax_noalloc(const double *q, int64_t n, double *result, hipStream_t s) {
+  double *partial = nullptr;
+  DGK_TRY(scratch_address(&partial));
+  // one block per 2048 values, at most kNoallocBlocks; blocks stride over the rest
+  const int64_t want = (n + 2047) / 2048;
+  const int nblk = (int)std::min<int64_t>(kNoallocBlocks, std::max<int64_t>(1, want));
+  return launch_minmax(q, n, partial, nblk, result, s);
 }
 
 int launch_init_gaussian(double *q, double *u, double *v, double *w, const double *x1D,

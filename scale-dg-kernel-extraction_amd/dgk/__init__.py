@@ -59,6 +59,7 @@ _SIGNATURES = {
     "dgk_rk_update": (_I, [_P, _I, _D]),
     "dgk_step": (_I, [_P, _D, _I]),
     "dgk_minmax": (_I, [_P, _P, _P]),
+    "dgk_minmax_noalloc": (_I, [_P, C.c_int64, _P, _P]),
     "dgk_calibrate_stream": (_I, [_L, _I, _P, _P]),
     "dgk_synchronize": (_I, [_P]),
     "dgk_set_kernel_timing": (_I, [_P, _I]),
