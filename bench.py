@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--batch-only", action="store_true",
                     help="only build the 512-instance ensemble in --form and run --steps "
                          "graph-replayed steps (for kernel traces of a batched step)")
+    ap.add_argument("--batch-orders", type=int, nargs="*", default=None, metavar="P",
+                    help="with --batch-only: keep only the ensemble's instances of these orders")
     return ap.parse_args()
 
 
@@ -206,16 +208,23 @@ def pcie_leg(dgk, inputs, form, steps):
 
 
 def batch_leg(dgk, inputs):
-    """Batched ensemble vs the plain FP64 path run instance by instance."""
+    """Batched ensemble (512 instances) in both forms: throughput, every
+    instance's mixed-precision error against its FP64 instance, and on the
+    p = 7 instances the decomposition of the batched mixed form's rate over
+    the plain FP64 path: batching, constant velocity read from the geometry
+    record instead of the fields (DGK_CVEL=0 keeps the field reads), and
+    precision."""
     ens = inputs.ensemble()
-    out = {"instances": len(ens), "nstep": ens[0]["nstep"]}
-    dof = np.array([(it["p"] + 1) ** 3 * it["ne"][0] ** 3 for it in ens], dtype=np.float64)
     nstep = ens[0]["nstep"]
-    for form in ("HIP", "HIP_MP"):
-        b = dgk.Batch(form, ens)
-        for k, it in enumerate(ens):
+    out = {"instances": len(ens), "nstep": nstep}
+    dof = np.array([(it["p"] + 1) ** 3 * it["ne"][0] ** 3 for it in ens], dtype=np.float64)
+    fields = {}
+
+    def run_batch(form, sub, keep=False):
+        b = dgk.Batch(form, sub)
+        for k, it in enumerate(sub):
             b.init_gaussian(k, it["centre"], it["sigma"], it["vel"])
-        out.setdefault("tile_shapes", {})[form] = b.autotune(reps=10)
+        tiles = b.autotune(reps=10)
         b.step(1)  # graph capture + warm-up
         b.synchronize()
         b.timer()
@@ -224,12 +233,28 @@ def batch_leg(dgk, inputs):
         b.synchronize()
         t1 = time.perf_counter()
         dev_s, _ = b.timer()
-        out[form] = {"DOF-updates/s": float(dof.sum() * nstep / (t1 - t0)),
-                     "seconds": t1 - t0, "device_seconds": dev_s,
-                     "device_bytes": b.size()["device_bytes"]}
+        got = [b.get_field(k) for k in range(len(sub))] if keep else None
+        rec = {"DOF-updates/s": float(sum((it["p"] + 1) ** 3 * it["ne"][0] ** 3 for it in sub)
+                                      * nstep / (t1 - t0)),
+               "seconds": t1 - t0, "device_seconds": dev_s,
+               "device_bytes": b.size()["device_bytes"], "tile_shapes": tiles}
         b.close()
-    # plain FP64 path, one solver per instance, on the p = 7 instances and a
-    # sample of the rest (extrapolated by DOF)
+        return rec, got
+
+    for form in ("HIP", "HIP_MP"):
+        out[form], fields[form] = run_batch(form, ens, keep=True)
+        out[form]["bytes_per_dof_update"] = 64
+    err = np.array([np.linalg.norm(fields["HIP_MP"][k] - fields["HIP"][k]) /
+                    np.linalg.norm(fields["HIP"][k]) for k in range(len(ens))])
+    worst = int(err.argmax())
+    out["per_instance_mp_vs_fp64"] = {
+        "instances": len(ens), "max_rel_l2": float(err.max()), "mean_rel_l2": float(err.mean()),
+        "worst_instance": worst, "worst_p": ens[worst]["p"], "worst_ne": ens[worst]["ne"][0],
+        "note": "every instance after 51 steps, mixed form against the FP64 form (the FP64 "
+                "instances equal their plain-path solves bit for bit, which the parity tests "
+                "hold to the oracle)"}
+    del fields
+    # plain FP64 path, one solver per instance, on the p = 7 instances
     p7 = [k for k, it in enumerate(ens) if it["p"] == 7]
     solo_s = 0.0
     for k in p7:
@@ -244,33 +269,43 @@ def batch_leg(dgk, inputs):
         solo_s += time.perf_counter() - t0
         s.close()
     p7_dof = float(dof[p7].sum())
-    out["plain_fp64_solo_p7"] = {"DOF-updates/s": p7_dof * nstep / solo_s, "seconds": solo_s,
-                                 "instances": len(p7)}
-    # batched forms on the p = 7 instances alone
+    solo = p7_dof * nstep / solo_s
+    out["plain_fp64_solo_p7"] = {"DOF-updates/s": solo, "seconds": solo_s, "instances": len(p7),
+                                 "bytes_per_dof_update": 136}
     sub = [ens[k] for k in p7]
-    for form in ("HIP", "HIP_MP"):
-        b = dgk.Batch(form, sub)
-        for k, it in enumerate(sub):
-            b.init_gaussian(k, it["centre"], it["sigma"], it["vel"])
-        b.autotune(reps=10)
-        b.step(1)
-        b.synchronize()
-        t0 = time.perf_counter()
-        b.step(nstep)
-        b.synchronize()
-        out[f"batched_{form}_p7"] = {"DOF-updates/s": p7_dof * nstep / (time.perf_counter() - t0)}
-        b.close()
-    out["speedup_batched_mp_vs_plain_fp64_p7"] = (
-        out["batched_HIP_MP_p7"]["DOF-updates/s"] / out["plain_fp64_solo_p7"]["DOF-updates/s"])
+    rates = {}
+    for form, cvel in (("HIP", "0"), ("HIP", "1"), ("HIP_MP", "0"), ("HIP_MP", "1")):
+        os.environ["DGK_CVEL"] = cvel
+        try:
+            rec, _ = run_batch(form, sub)
+        finally:
+            os.environ.pop("DGK_CVEL", None)
+        key = f"batched_{form}_p7" + ("" if cvel == "1" else "_velocity_fields")
+        rec["bytes_per_dof_update"] = 64 if cvel == "1" else (136 if form == "HIP" else 100)
+        rec.pop("tile_shapes", None)
+        out[key] = rec
+        rates[(form, cvel)] = rec["DOF-updates/s"]
+    out["decomposition_p7"] = {
+        "batching (plain FP64 solo -> batched FP64, velocity fields)": rates[("HIP", "0")] / solo,
+        "constant velocity (batched FP64: fields -> geometry record)": rates[("HIP", "1")] / rates[("HIP", "0")],
+        "precision (batched, constant velocity: FP64 -> mixed)": rates[("HIP_MP", "1")] / rates[("HIP", "1")],
+        "precision with velocity fields (batched: FP64 -> mixed)": rates[("HIP_MP", "0")] / rates[("HIP", "0")],
+        "byte-model cap of the precision gain with velocity fields (136 B -> 100 B)": 136 / 100,
+        "batched mixed (constant velocity) / plain FP64 solo": rates[("HIP_MP", "1")] / solo,
+    }
+    out["speedup_batched_mp_vs_plain_fp64_p7"] = rates[("HIP_MP", "1")] / solo
     return out
 
 
-def batch_only(dgk, form, steps):
-    """The 512-instance ensemble in one form: autotune, capture, then `steps`
-    graph-replayed steps, nothing else (the command kernel traces wrap)."""
+def batch_only(dgk, form, steps, orders=None):
+    """The 512-instance ensemble (or its instances of the given orders) in one
+    form: autotune, capture, then `steps` graph-replayed steps, nothing else
+    (the command kernel traces wrap)."""
     from dgk import inputs
 
     ens = inputs.ensemble()
+    if orders:
+        ens = [it for it in ens if it["p"] in orders]
     with dgk.Batch(form, ens) as b:
         for k, it in enumerate(ens):
             b.init_gaussian(k, it["centre"], it["sigma"], it["vel"])
@@ -431,7 +466,7 @@ def main():
         print(json.dumps({"stream_GBs": a, "stream_regs_GBs": r}), flush=True)
         return
     if args.batch_only:
-        batch_only(dgk, args.form, args.steps)
+        batch_only(dgk, args.form, args.steps, args.batch_orders)
         return
     if args.per_order_only:
         per_order_only(dgk, args.form, args.per_order_only, args.steps)
@@ -484,7 +519,11 @@ def main():
             "achieved_GBs": BYTES_PER_DOF_UPDATE[other] / 3.0 * dof / r2["launch_s"] / 1e9,
             "ratio_vs_" + args.form: r2["value"] / r["value"]}
         line["extra"]["pcie_inclusive"] = pcie_leg(dgk, inputs, args.form, args.steps)
-        line["extra"]["batched_ensemble"] = batch_leg(dgk, inputs)
+        be = batch_leg(dgk, inputs)
+        be["decomposition_p7"]["batched mixed (constant velocity) / plain FP64 headline "
+                               "(one 32^3 p=7 mesh, velocity fields)"] = (
+            be["batched_HIP_MP_p7"]["DOF-updates/s"] / line["value"])
+        line["extra"]["batched_ensemble"] = be
         line["extra"]["c3_p15"] = c3_leg(dgk, inputs)
         line["extra"]["per_order"] = per_order_leg(dgk)
     if args.serve:

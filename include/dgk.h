@@ -215,7 +215,12 @@ int dgk_timer_reset(dgk_solver *s);
  * advanced together by the stage kernels (one launch per stage and order);
  * dgk_batch_step captures one SSP-RK3 step of every order group in a
  * hipGraph on first use and replays it nstep times.  Per-instance arithmetic
- * is identical to a dgk_solver of that instance alone. */
+ * is identical to a dgk_solver of that instance alone.  A velocity component
+ * that is one constant over an instance (dgk_batch_init_gaussian, or a
+ * constant field given to dgk_batch_set_field) is kept in the instance's
+ * geometry record; an order group whose instances all have constant
+ * velocities runs stage kernels that read it there instead of the fields
+ * (the same bits, 24 bytes less per DOF and stage). */
 typedef struct dgk_batch dgk_batch;
 int dgk_batch_create(int evaltype, int ninst, const int *ne_xyz, const int *p,
                      const double *dom_xyz, const double *dt,
@@ -295,9 +300,12 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
  * at most 8 times, after which admission waits for it.  Every tick advances
  * each admitted request by SSP-RK3 steps on its stream (see
  * dgk_serve_set_policy), so a request that arrives while others run joins at
- * the next tick and its kernels overlap theirs.  A request runs the same stage kernels on the same inputs as a
- * dgk_solver of its box, so its result does not depend on what else is
- * admitted.  All dgk_serve_* calls may come from any thread. */
+ * the next tick and its kernels overlap theirs.  A request runs the same
+ * stage arithmetic on the same inputs as a dgk_solver of its box, so its
+ * result does not depend on what else is admitted.  Its arena block holds
+ * q only: the velocity is one constant per request, which the stage kernels
+ * read from the request's geometry record.  All dgk_serve_* calls may come
+ * from any thread. */
 enum { DGK_TOL_FP64 = 0, /* plain FP64 path (HIP)                         */
        DGK_TOL_1E5 = 1   /* mixed precision (HIP_MP), 1e-5 relative L2     */ };
 

@@ -153,6 +153,7 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
   a.dt_geo = 0;
   a.tile = s->tile;
   a.geo_per_run = s->geo_per_run;
+  a.cvel = s->cvel;
   return a;
 }
 
@@ -173,6 +174,7 @@ int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out
     a.dt_geo = dt_geo;
     a.tile = s->tile;
     a.geo_per_run = s->geo_per_run;
+    a.cvel = s->cvel;
     return launch_stage_mp(s->Nq, a, stream);
   }
   StageArgs a = stage_args(s, qin, q0, out, mode, ca, cb, dt);
@@ -697,7 +699,12 @@ struct dgk_batch {
   hipStream_t stream = nullptr;  // capture origin
   std::vector<dgk_solver *> groups;
   std::vector<int> group_p;
-  struct Loc { int group; int64_t elem_off; int64_t Ne; int p; BoxShape box; };
+  // rec: the instance's geometry record in its group; cv[d]: velocity
+  // component d is one constant, held at word kGeoVel + d of that record
+  struct Loc {
+    int group; int64_t elem_off; int64_t Ne; int p; BoxShape box;
+    int rec = 0; bool cv[3] = {false, false, false};
+  };
   std::vector<Loc> loc;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
@@ -804,6 +811,34 @@ int batch_fold_timers(dgk_batch *b, bool wait) {
   return DGK_OK;
 }
 
+// Record instance inst's constant velocity components (vel[d], for the d with
+// set[d]) in its geometry record, or mark them non-constant (set[d] and not
+// constant[d]); a group launches its constant-velocity kernels when every one
+// of its boxes has all three components constant.  A change of the group's
+// form releases the captured steps.
+int batch_set_velocity(dgk_batch *b, int inst, const bool set[3], const bool constant[3],
+                       const double vel[3]) {
+  dgk_batch::Loc &l = b->loc[inst];
+  dgk_solver *s = b->groups[l.group];
+  for (int d = 0; d < 3; ++d) {
+    if (!set[d]) continue;
+    l.cv[d] = constant[d];
+    if (constant[d])
+      DGK_HIP(hipMemcpy(s->geo + (int64_t)l.rec * kGeoStride + kGeoVel + d, &vel[d],
+                        sizeof(double), hipMemcpyHostToDevice));
+  }
+  // DGK_CVEL=0 keeps the per-node velocity reads (for A/B timings)
+  const char *env = std::getenv("DGK_CVEL");
+  int all = (s->axis_aligned && !(env && env[0] == '0')) ? 1 : 0;
+  for (const auto &m : b->loc)
+    if (m.group == l.group && !(m.cv[0] && m.cv[1] && m.cv[2])) all = 0;
+  if (all != s->cvel) {
+    s->cvel = all;
+    batch_release_graphs(b);
+  }
+  return DGK_OK;
+}
+
 int batch_loc(const dgk_batch *b, int inst, const dgk_batch::Loc **l) {
   if (inst < 0 || inst >= (int)b->loc.size())
     return set_error(DGK_ERR_ARG, "instance index " + std::to_string(inst) + " out of range");
@@ -889,7 +924,12 @@ int dgk_batch_create(int evaltype, int ninst, const int *ne_xyz, const int *p,
       BoxShape bs;
       bs.NeX = inst[j].NeX; bs.NeY = inst[j].NeY; bs.NeZ = inst[j].NeZ;
       for (int d = 0; d < 3; ++d) bs.dom[d] = inst[j].dom[d];
-      b->loc[k] = {(int)g, off[j], off[j + 1] - off[j], b->group_p[g], bs};
+      b->loc[k].group = (int)g;
+      b->loc[k].elem_off = off[j];
+      b->loc[k].Ne = off[j + 1] - off[j];
+      b->loc[k].p = b->group_p[g];
+      b->loc[k].box = bs;
+      b->loc[k].rec = (int)j;
     }
     s->set[DGK_FIELD_Q] = s->set[DGK_FIELD_U] = s->set[DGK_FIELD_V] = s->set[DGK_FIELD_W] = true;
   }
@@ -945,6 +985,19 @@ int dgk_batch_set_field(dgk_batch *b, int inst, int which, const double *f, int6
     DGK_TRY(launch_to_f32(dst, d32, cnt, s->stream));
   }
   DGK_HIP(hipStreamSynchronize(s->stream));
+  if (which != DGK_FIELD_Q) {
+    // a velocity component that is one constant over the box is read from
+    // the instance's geometry record instead of the field
+    const int d = which - DGK_FIELD_U;
+    bool set[3] = {false, false, false}, constant[3] = {false, false, false};
+    double vel[3] = {0.0, 0.0, 0.0};
+    set[d] = true;
+    constant[d] = std::all_of(f, f + cnt, [&](double x) {
+      return std::memcmp(&x, f, sizeof(double)) == 0;
+    });
+    vel[d] = f[0];
+    DGK_TRY(batch_set_velocity(b, inst, set, constant, vel));
+  }
   return DGK_OK;
 }
 
@@ -985,7 +1038,8 @@ int dgk_batch_init_gaussian(dgk_batch *b, int inst, double cx, double cy, double
     DGK_TRY(launch_to_f32(s->w + off, s->w32 + off, cnt, s->stream));
   }
   DGK_HIP(hipStreamSynchronize(s->stream));
-  return DGK_OK;
+  const bool all[3] = {true, true, true};
+  return batch_set_velocity(b, inst, all, all, vel);
 }
 
 int dgk_batch_set_graph(dgk_batch *b, int on) {

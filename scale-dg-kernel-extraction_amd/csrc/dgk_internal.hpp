@@ -40,11 +40,13 @@ constexpr int kMaxNqValu = 16;  // the VALU FP64 kernels; above, MFMA kernels on
 // Per-element geometry record: Escale[3] (2/dx_d, constant per element,
 // mod_mesh.f90:147-149), Fscale[6] and the outward normal of each face
 // (set_face_geometry, mod_mesh.f90:356-397).  Identical records are shared.
-constexpr int kGeoStride = 32;  // doubles per record (27 used)
+constexpr int kGeoStride = 32;  // doubles per record (31 used)
 constexpr int kGeoEscale = 0;
 constexpr int kGeoFscale = 3;
 constexpr int kGeoNormal = 9;  // + 3*f + d
 constexpr int kGeoDt = 27;     // time step of the element's instance (batched solves)
+constexpr int kGeoVel = 28;    // constant velocity u, v, w of the element's box (28..30),
+                               // read instead of the fields by launches with cvel set
 
 // Node index (0-based) inside an element of face node (a,b) of face f on the
 // element's own side, in the reference's Fmask order (mod_mesh.f90:278-324):
@@ -225,6 +227,7 @@ struct StageArgs {
   int tile;               // tile-shape candidate, -1: the process-wide selection
   int geo_per_run = 0;    // with geoid: every run of run_elements(Nq) elements from a
                           // multiple of it shares one geometry record
+  int cvel = 0;           // velocity constant per element, from its geometry record
 };
 
 // Mixed-precision operands (FP32 copies of the constant inputs).
@@ -249,6 +252,7 @@ struct StageArgsMP {
   int dt_geo;
   int tile;               // tile-shape candidate, -1: the process-wide selection
   int geo_per_run = 0;    // as StageArgs::geo_per_run, for the mixed run kernel
+  int cvel = 0;           // as StageArgs::cvel
 };
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);

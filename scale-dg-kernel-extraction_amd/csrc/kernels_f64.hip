@@ -93,7 +93,11 @@ struct Staged {
 // lets the axis-aligned kernel drop them (p = 1: 85 -> 62 VGPRs, 5 -> 8 waves
 // per SIMD).  The register form at p = 11 keeps the test at run time: with
 // it decided at compile time the allocation lands at 168 VGPRs with 2 spilled.
-template <int NQ, bool VREG, bool AXAL>
+// CVEL: every element's velocity is constant, (u, v, w) = words kGeoVel..+2
+// of its geometry record (batched and served boxes); no velocity field is
+// read.  The constants enter the same expressions as the per-node values
+// they equal, so the results are the same bits.
+template <int NQ, bool VREG, bool AXAL, bool CVEL>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, VREG>::THREADS),
                                amdgpu_waves_per_eu(Staged<NQ, VREG>::MIN_WAVES_EU)))
 stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
@@ -107,6 +111,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
           const int64_t zp_) {
   // Read-only operands are __restrict__ kernel parameters, so wave-uniform
   // reads of them (operators, geometry, neighbour table) are scalar loads.
+  static_assert(!CVEL || AXAL, "constant velocity on axis-aligned faces");
   using C = Staged<NQ, VREG>;
   constexpr int NQ2 = C::NQ2, NP = C::NP, EPB = C::EPB, NFT = 6 * NQ2;
   extern __shared__ double smem[];
@@ -140,6 +145,12 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   const int lb = el * NP;  // element block in the staged fields
   const bool rk = mode_ == 1;
   const bool use_q0 = rk && ca_ != 0.0;
+  double cvel[3] = {0.0, 0.0, 0.0};
+  if constexpr (CVEL) {
+    cvel[0] = g[kGeoVel];
+    cvel[1] = g[kGeoVel + 1];
+    cvel[2] = g[kGeoVel + 2];
+  }
 
   // With several elements per workgroup the neighbour-table entries are
   // per-lane loads: issued first, so waiting for them before the trace
@@ -166,7 +177,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
         int64_t off = 0;
         if (del < EPB && sl < Ne_) off = slot_element(sl, Ne_, order_, rev_, zp_) * NP + dn;
 #pragma unroll
-        for (int fld = 0; fld < C::NSTAGED; ++fld) {
+        for (int fld = 0; fld < (CVEL ? 1 : C::NSTAGED); ++fld) {
           const char *gp = (const char *)(src[fld] + off) + (C::VEC16 ? 0 : 4 * (lane & 1));
           __attribute__((address_space(3))) void *lp =
               (__attribute__((address_space(3))) void *)(smem + fld * C::FIELD + c * C::CHUNK);
@@ -201,7 +212,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
         const int64_t pidx = nbr_at(f);
         const double *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
         qP[f] = q_[pidx];
-        uP[f] = vel[pidx];
+        uP[f] = CVEL ? cvel[kFaceAxis[f]] : vel[pidx];
       }
     } else {
 #pragma unroll
@@ -219,11 +230,11 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   if (VREG && active) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      vc[k] = v_[base + (int64_t)k * NQ2];
-      wc[k] = w_[base + (int64_t)k * NQ2];
+      vc[k] = CVEL ? cvel[1] : v_[base + (int64_t)k * NQ2];
+      wc[k] = CVEL ? cvel[2] : w_[base + (int64_t)k * NQ2];
     }
-    vM[0] = v_[e * NP + own_face_node(0, i, j, NQ)];
-    vM[1] = v_[e * NP + own_face_node(2, i, j, NQ)];
+    vM[0] = CVEL ? cvel[1] : v_[e * NP + own_face_node(0, i, j, NQ)];
+    vM[1] = CVEL ? cvel[1] : v_[e * NP + own_face_node(2, i, j, NQ)];
   }
   // the RK base and the output are touched once per launch: non-temporal,
   // so L2 keeps the neighbour traces
@@ -253,7 +264,9 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
         double VelM, VelP;
         if (axal) {
           double vm;
-          if constexpr (VREG)
+          if constexpr (CVEL)
+            vm = cvel[kFaceAxis[f]];
+          else if constexpr (VREG)
             vm = kFaceAxis[f] == 0 ? su[m] : f == 0 ? vM[0] : f == 2 ? vM[1]
                  : f == 4 ? wc[0] : wc[NQ - 1];
           else
@@ -285,8 +298,9 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const int n = lb + ij + k * NQ2;
-      const double qq = sq[n], uu = su[n];
-      const double vv = VREG ? vc[k] : sv[n], ww = VREG ? wc[k] : sw[n];
+      const double qq = sq[n], uu = CVEL ? cvel[0] : su[n];
+      const double vv = CVEL ? cvel[1] : VREG ? vc[k] : sv[n];
+      const double ww = CVEL ? cvel[2] : VREG ? wc[k] : sw[n];
       qc[k] = qq;
       fz[k] = qq * ww;
       if (C::FX_T) su[lb + j + i * NQ + k * NQ2] = qq * uu;
@@ -466,14 +480,14 @@ stage_f64_lift(const StageArgs P) {
   }
 }
 
-template <int NQ, bool VREG, bool AXAL>
+template <int NQ, bool VREG, bool AXAL, bool CVEL = false>
 int prepare_shape_valu() {
   // set once per device; concurrent callers may both set it (idempotent)
   static std::atomic<int> prepared_dev{-1};
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev.load(std::memory_order_acquire) != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG, AXAL>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG, AXAL, CVEL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)Staged<NQ, VREG>::LDS_BYTES));
     prepared_dev.store(dev, std::memory_order_release);
@@ -486,16 +500,18 @@ int prepare_nq() {
   DGK_TRY((prepare_shape_valu<NQ, false, false>()));
   DGK_TRY((prepare_shape_valu<NQ, false, true>()));
   DGK_TRY((prepare_shape_valu<NQ, true, false>()));
+  DGK_TRY((prepare_shape_valu<NQ, false, true, true>()));
+  DGK_TRY((prepare_shape_valu<NQ, true, true, true>()));
   return prepare_shape_valu<NQ, true, true>();
 }
 
-template <int NQ, bool VREG, bool AXAL>
+template <int NQ, bool VREG, bool AXAL, bool CVEL = false>
 int launch_valu(const StageArgs &a, hipStream_t s) {
   using T = Staged<NQ, VREG>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  DGK_TRY((prepare_shape_valu<NQ, VREG, AXAL>()));
-  hipLaunchKernelGGL((stage_f64<NQ, VREG, AXAL>), dim3((unsigned)nblk), dim3(T::THREADS),
+  DGK_TRY((prepare_shape_valu<NQ, VREG, AXAL, CVEL>()));
+  hipLaunchKernelGGL((stage_f64<NQ, VREG, AXAL, CVEL>), dim3((unsigned)nblk), dim3(T::THREADS),
                      T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr,
                      a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt,
                      a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
@@ -509,6 +525,8 @@ template <int NQ>
 int launch_nq(const StageArgs &a, hipStream_t s, bool vreg) {
   if (a.ell) {
     const bool ax = a.axis_aligned != 0;
+    if (a.cvel && ax && !a.flux_in && !a.vmapP)
+      return vreg ? launch_valu<NQ, true, true, true>(a, s) : launch_valu<NQ, false, true, true>(a, s);
     if (vreg) return ax ? launch_valu<NQ, true, true>(a, s) : launch_valu<NQ, true, false>(a, s);
     return ax ? launch_valu<NQ, false, true>(a, s) : launch_valu<NQ, false, false>(a, s);
   }
@@ -636,9 +654,11 @@ __global__ void init_gaussian(double *q, double *u, double *v, double *w,
   d = y - cy; d = d - rint(d); r2 += d * d;
   d = z - cz; d = d - rint(d); r2 += d * d;
   q[t] = exp(-r2 / (2.0 * sigma * sigma));
-  u[t] = ux;
-  v[t] = uy;
-  w[t] = uz;
+  if (u) {  // no velocity fields when the launches read the constant itself
+    u[t] = ux;
+    v[t] = uy;
+    w[t] = uz;
+  }
 }
 
 __global__ void to_f32(const double *__restrict__ src, float *__restrict__ dst,

@@ -206,7 +206,11 @@ __device__ __forceinline__ void mfma_tile(typename P::op_t *tile,
 // allocations grow: the direction-by-direction (SEQ) shapes (p = 5..9:
 // 5 -> 4 waves per SIMD) and the mixed one-wavefront shapes held to 4 waves
 // (p = 2..4 then spill 2 to 24 VGPRs).
-template <class P, int NQ, int EPB, bool SEQ, bool AXAL>
+// CVEL: constant velocity per element, (u, v, w) = words kGeoVel..+2 of its
+// geometry record (batched and served boxes); no velocity field is read, and
+// the operand-precision value of a constant is the value its per-node copy
+// holds, so the results are the same bits.
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL = false>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, MP<P, NQ, EPB, SEQ>::THREADS),
                                amdgpu_waves_per_eu(MP<P, NQ, EPB, SEQ>::MIN_WAVES_EU)))
 stage_mfma(const double *__restrict__ q_, const double *q0_,
@@ -252,6 +256,12 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   const bool axal = C::AX_RUNTIME ? axis_aligned_ != 0 : AXAL;
   const int R = el * NQ2;
   const vel_t *const velp[3] = {u_, v_, w_};
+  vel_t cv[3] = {(vel_t)0, (vel_t)0, (vel_t)0};
+  if constexpr (CVEL) {
+    cv[0] = (vel_t)g[kGeoVel];
+    cv[1] = (vel_t)g[kGeoVel + 1];
+    cv[2] = (vel_t)g[kGeoVel + 2];
+  }
 
   // ---- 1. loads straight into registers ------------------------------------
   // Neighbour-table entries: with several elements per workgroup they are
@@ -278,28 +288,28 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
     for (int k = 0; k < NQ; ++k) {
       qc[k] = q_[base + k * NQ2];
       if constexpr (!SEQ) {
-        uc[k] = u_[base + k * NQ2];
-        vc[k] = v_[base + k * NQ2];
-        wc[k] = w_[base + k * NQ2];
+        uc[k] = CVEL ? cv[0] : u_[base + k * NQ2];
+        vc[k] = CVEL ? cv[1] : v_[base + k * NQ2];
+        wc[k] = CVEL ? cv[2] : w_[base + k * NQ2];
       }
     }
     if constexpr (SEQ) {
-      wc[0] = w_[base];
-      wc[NQ - 1] = w_[base + (NQ - 1) * NQ2];
+      wc[0] = CVEL ? cv[2] : w_[base];
+      wc[NQ - 1] = CVEL ? cv[2] : w_[base + (NQ - 1) * NQ2];
     }
     if constexpr (!C::XCH && !C::XD) {
 #pragma unroll
       for (int f = 0; f < 4; ++f) {  // own traces of the x and y faces
         const int64_t m = ebase + own_face_node(f, i, j, NQ);
         qM[f] = q_[m];
-        vM[f] = (kFaceAxis[f] == 0 ? u_ : v_)[m];
+        vM[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? u_ : v_)[m];
       }
     }
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       const int64_t pidx = nbr_at(f);
       qP[f] = q_[pidx];
-      vP[f] = (kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_)[pidx];
+      vP[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_)[pidx];
     }
   }
 
@@ -315,8 +325,10 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
         xq[tid * NQ + k] = qc[k];
-        xu[tid * NQ + k] = uc[k];
-        xv[tid * NQ + k] = vc[k];
+        if constexpr (!CVEL) {
+          xu[tid * NQ + k] = uc[k];
+          xv[tid * NQ + k] = vc[k];
+        }
       }
     }
     __syncthreads();
@@ -325,7 +337,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         qM[f] = xq[src[f] * NQ + j];
-        vM[f] = (kFaceAxis[f] == 0 ? xu : xv)[src[f] * NQ + j];
+        vM[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? xu : xv)[src[f] * NQ + j];
       }
     }
     __syncthreads();  // the A operands below reuse the area
@@ -362,8 +374,15 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
         const int64_t pidx = nbr_at(f);
         const double n1 = g[kGeoNormal + 3 * f], n2 = g[kGeoNormal + 3 * f + 1],
                      n3 = g[kGeoNormal + 3 * f + 2];
-        VelM = (double)u_[m] * n1 + (double)v_[m] * n2 + (double)w_[m] * n3;
-        VelP = (double)u_[pidx] * n1 + (double)v_[pidx] * n2 + (double)w_[pidx] * n3;
+        if constexpr (CVEL) {
+          VelM = (double)cv[0] * n1 + (double)cv[1] * n2 + (double)cv[2] * n3;
+          VelP = VelM;
+          (void)m;
+          (void)pidx;
+        } else {
+          VelM = (double)u_[m] * n1 + (double)v_[m] * n2 + (double)w_[m] * n3;
+          VelP = (double)u_[pidx] * n1 + (double)v_[pidx] * n2 + (double)w_[pidx] * n3;
+        }
       }
       const double alpha = 0.5 * fabs(VelP + VelM);
       fl[f] = (op_t)(0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qm * VelM - alpha * (qP[f] - qm)));
@@ -433,7 +452,7 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
           qM[f] = xd_q[f * NQ2 + ij];
-          vM[f] = xd_v[f * NQ2 + ij];
+          vM[f] = CVEL ? cv[kFaceAxis[f]] : xd_v[f * NQ2 + ij];
           face_flux(f);
         }
 #pragma unroll
@@ -476,7 +495,8 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
         op_t *a = As + pos_off[d];
 #pragma unroll
         for (int k = 0; k < NQ; ++k)
-          a[k * pos_step[d]] = (op_t)(Ed * (qc[k] * (double)vel[base + k * NQ2]));
+          a[k * pos_step[d]] =
+              (op_t)(Ed * (qc[k] * (double)(CVEL ? cv[d] : vel[base + k * NQ2])));
         op_t *r = As + (R + ij) * KPAD;
         r[NQ] = fl[face_m[d]];
         r[NQ + 1] = fl[face_p[d]];
@@ -830,10 +850,10 @@ bool aligned16_mp(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // The mixed run kernel takes the launch (true), or the launch needs another
 // kernel (false): node connectivity, an order array, per-element geometry
-// within a tile, or unaligned fields.
+// within a tile, unaligned fields, or constant velocities (no fields).
 bool run_mp_applicable(int Nq, const StageArgsMP &a) {
   const int T = run_mp_elements(Nq);
-  if (T < 1 || !a.nbr || a.vmapP || a.order) return false;
+  if (T < 1 || !a.nbr || a.vmapP || a.order || a.cvel) return false;
   if (a.geoid && !a.geo_per_run) return false;
   const bool q0_ok = a.mode != 1 || a.ca == 0.0 || aligned16_mp(a.q0);
   return aligned16_mp(a.q) && aligned16_mp(a.u) && aligned16_mp(a.v) && aligned16_mp(a.w) &&
@@ -851,14 +871,14 @@ int launch_run_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   }
 }
 
-template <class P, int NQ, int EPB, bool SEQ, bool AXAL>
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL = false>
 int prepare_form() {
   // set once per device; concurrent callers may both set it (idempotent)
   static std::atomic<int> prepared_dev{-1};
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev.load(std::memory_order_acquire) != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ, AXAL>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ, AXAL, CVEL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)MP<P, NQ, EPB, SEQ>::LDS_BYTES));
     prepared_dev.store(dev, std::memory_order_release);
@@ -870,18 +890,20 @@ int prepare_form() {
 // AX_RUNTIME shapes).
 template <class P, int NQ, int EPB, bool SEQ>
 int prepare_shape() {
+  constexpr bool AXR = MP<P, NQ, EPB, SEQ>::AX_RUNTIME;
   DGK_TRY((prepare_form<P, NQ, EPB, SEQ, false>()));
-  if constexpr (!MP<P, NQ, EPB, SEQ>::AX_RUNTIME) DGK_TRY((prepare_form<P, NQ, EPB, SEQ, true>()));
+  if constexpr (!AXR) DGK_TRY((prepare_form<P, NQ, EPB, SEQ, true>()));
+  DGK_TRY((prepare_form<P, NQ, EPB, SEQ, !AXR, true>()));
   return DGK_OK;
 }
 
-template <class P, int NQ, int EPB, bool SEQ, bool AXAL, class Args>
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, class Args, bool CVEL = false>
 int launch_form(const Args &a, hipStream_t s) {
   using T = MP<P, NQ, EPB, SEQ>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  DGK_TRY((prepare_form<P, NQ, EPB, SEQ, AXAL>()));
-  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ, AXAL>), dim3((unsigned)nblk),
+  DGK_TRY((prepare_form<P, NQ, EPB, SEQ, AXAL, CVEL>()));
+  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ, AXAL, CVEL>), dim3((unsigned)nblk),
                      dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr,
                      a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt,
                      a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
@@ -891,7 +913,11 @@ int launch_form(const Args &a, hipStream_t s) {
 
 template <class P, int NQ, int EPB, bool SEQ, class Args>
 int launch_shape(const Args &a, hipStream_t s) {
-  if constexpr (!MP<P, NQ, EPB, SEQ>::AX_RUNTIME)
+  constexpr bool AXR = MP<P, NQ, EPB, SEQ>::AX_RUNTIME;
+  // constant velocity: batched and served boxes, whose faces are axis-aligned
+  if (a.cvel && a.axis_aligned && !a.vmapP)
+    return launch_form<P, NQ, EPB, SEQ, !AXR, Args, true>(a, s);
+  if constexpr (!AXR)
     if (a.axis_aligned) return launch_form<P, NQ, EPB, SEQ, true>(a, s);
   return launch_form<P, NQ, EPB, SEQ, false>(a, s);
 }

@@ -25,7 +25,8 @@
 //
 // Requirements (checked by run_applicable on the host): the compact
 // neighbour table, the separable Lift, no given flux, no order array, field
-// pointers 16-byte aligned, and one geometry record per tile.
+// pointers 16-byte aligned, one geometry record per tile, and velocity
+// fields (launches with constant velocities run the VALU kernel).
 #include <algorithm>
 
 #include "dgk_internal.hpp"
@@ -315,7 +316,7 @@ int run_elements(int Nq) {
 
 bool run_applicable(int Nq, const StageArgs &a) {
   const int T = run_elements(Nq);
-  if (T < 1 || !a.ell || a.flux_in || !a.nbr || a.vmapP || a.order) return false;
+  if (T < 1 || !a.ell || a.flux_in || !a.nbr || a.vmapP || a.order || a.cvel) return false;
   if (a.geoid && !a.geo_per_run) return false;
   const bool q0_ok = a.mode != 1 || a.ca == 0.0 || aligned16(a.q0);
   return aligned16(a.q) && aligned16(a.u) && aligned16(a.v) && aligned16(a.w) &&

@@ -124,15 +124,11 @@ Layout layout(int64_t Ne, int Np, bool mp) {
     return o;
   };
   const int64_t dof = Ne * Np;
+  // q in three rotation buffers; the velocity is one constant per request,
+  // read by the stage kernels from the geometry record, so no velocity fields
+  (void)mp;
   for (int b = 0; b < 3; ++b) L.q[b] = take(dof * 8);
-  L.u = take(dof * 8);
-  L.v = take(dof * 8);
-  L.w = take(dof * 8);
-  if (mp) {
-    L.u32 = take(dof * 4);
-    L.v32 = take(dof * 4);
-    L.w32 = take(dof * 4);
-  }
+  L.u = L.v = L.w = -1;
   L.nbr = take(Ne * 6 * 4);
   L.geo = take(kGeoStride * 8);
   L.sum = take((4 * kSummaryBlocks + 4) * 8);
@@ -301,14 +297,9 @@ int start(dgk_server *sv, Req *r, int64_t off) {
   s.NhaloNode = 0;
   s.nfield = r->dof;
   for (int b = 0; b < 3; ++b) s.q[b] = (double *)(blk + L.q[b]);
-  s.u = (double *)(blk + L.u);
-  s.v = (double *)(blk + L.v);
-  s.w = (double *)(blk + L.w);
-  if (mp) {
-    s.u32 = (float *)(blk + L.u32);
-    s.v32 = (float *)(blk + L.v32);
-    s.w32 = (float *)(blk + L.w32);
-  }
+  s.u = s.v = s.w = nullptr;
+  s.u32 = s.v32 = s.w32 = nullptr;
+  s.cvel = 1;  // the stage kernels read the velocity from the geometry record
   s.nbr = (int32_t *)(blk + L.nbr);
   s.geo = (double *)(blk + L.geo);
   r->sum_dev = (double *)(blk + L.sum);
@@ -333,16 +324,12 @@ int start(dgk_server *sv, Req *r, int64_t off) {
   const double h[3] = {r->r.dom[0] / ne[0], r->r.dom[1] / ne[1], r->r.dom[2] / ne[2]};
   double *slot = sv->pinned + (int64_t)r->slot * kSlotDoubles;
   fill_box_geo(h[0], h[1], h[2], slot + 4);
+  for (int d = 0; d < 3; ++d) slot[4 + kGeoVel + d] = r->r.vel[d];
   DGK_HIP(hipMemcpyAsync(s.geo, slot + 4, sizeof(double) * kGeoStride, hipMemcpyHostToDevice,
                          stream));
   DGK_TRY(launch_box_nbr(s.nbr, ne[0], ne[1], ne[2], 0, stream));
-  DGK_TRY(launch_init_gaussian(s.q[0], s.u, s.v, s.w, op.x1D, r->Nq, ne[0], ne[1], ne[2], h,
-                               r->r.centre, r->r.sigma, r->r.vel, stream));
-  if (mp) {
-    DGK_TRY(launch_to_f32(s.u, s.u32, r->dof, stream));
-    DGK_TRY(launch_to_f32(s.v, s.v32, r->dof, stream));
-    DGK_TRY(launch_to_f32(s.w, s.w32, r->dof, stream));
-  }
+  DGK_TRY(launch_init_gaussian(s.q[0], nullptr, nullptr, nullptr, op.x1D, r->Nq, ne[0], ne[1],
+                               ne[2], h, r->r.centre, r->r.sigma, r->r.vel, stream));
   r->steps_left = r->r.nstep;
   r->t_admit = seconds_since(sv->t0);
   sv->active.push_back(r);
@@ -688,10 +675,11 @@ int dgk_serve_submit(dgk_server *sv, const dgk_request *r, int64_t *id) {
   auto *q = new Req();
   q->r = *r;
   q->evaltype = r->tol_class == DGK_TOL_1E5 ? DGK_EVAL_HIP_MP : DGK_EVAL_HIP;
-  if (q->evaltype == DGK_EVAL_HIP_MP && !op.separable) {
+  if ((q->evaltype == DGK_EVAL_HIP_MP && !op.separable) ||
+      (q->evaltype == DGK_EVAL_HIP && !op.use_ell)) {
     delete q;
     return set_error(DGK_ERR_UNSUPPORTED, "the Lift of this order is not of tensor-product "
-                                          "form; the 1e-5 class needs it");
+                                          "form; served solves need it");
   }
   q->Nq = r->p + 1;
   q->Np = q->Nq * q->Nq * q->Nq;

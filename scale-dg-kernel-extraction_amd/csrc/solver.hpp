@@ -42,6 +42,8 @@ struct dgk_solver {
   int axis_aligned = 0;
   int tile = -1;             // tile-shape candidate chosen for this mesh (-1: process default)
   int geo_per_run = 1;       // every run of run_elements(Nq) elements shares a geometry record
+  int cvel = 0;              // stage launches read each element's constant velocity from its
+                             // geometry record (batch groups whose boxes all have one)
   double *x1D = nullptr;     // LGL nodes (device), for device-side initial conditions
   BoxShape box;              // structured box (NeX == 0 when built from arrays)
   int32_t *nbr = nullptr, *vmapP = nullptr, *halo = nullptr;

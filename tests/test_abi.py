@@ -107,13 +107,13 @@ def test_input_generator_matches_oracle_stream():
 def test_serve_footprint_and_no_device():
     from dgk import serving
 
-    # q (3 buffers) + u, v, w in FP64, plus FP32 velocity copies for the 1e-5
-    # class, neighbour table, geometry record and summary partials, rounded
-    # to 64 KiB blocks
+    # q (3 buffers; the velocity is one constant per request, held in the
+    # geometry record), neighbour table, geometry record and summary
+    # partials, rounded to 64 KiB blocks, the same for both classes
     for ne, p, cls in (((4, 4, 4), 2, 0), ((32, 32, 32), 15, 1), ((5, 7, 3), 7, 0)):
         ne_tot = int(np.prod(ne))
         dof = ne_tot * (p + 1) ** 3
-        fields = 6 * 8 * dof + (3 * 4 * dof if cls else 0)
+        fields = 3 * 8 * dof
         got = serving.footprint(ne, p, cls)
         assert got % 65536 == 0
         assert fields + 24 * ne_tot <= got <= fields + 24 * ne_tot + 8 * 256 + 8224 + 65536 + 256 * 10

@@ -91,3 +91,29 @@ def test_small_instances_against_oracle():
             ref, _, _ = m.run(*init[k], it["dt"], it["nstep"])
             got = b.get_field(k)
             assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), k
+
+
+@pytest.mark.parametrize("form", ["HIP", "HIP_MP"])
+def test_constant_and_field_velocities(form):
+    """Boxes set up by init_gaussian have one constant velocity, which their
+    order group reads from the geometry records.  A varying velocity field
+    given to one instance puts its group back on the fields; a constant
+    field given afterwards puts it back on the records.  Every instance
+    equals its solve alone, bit for bit, in each state."""
+    ens = small_ensemble(n=6, seed=16)
+    it0 = ens[0]
+    n0 = int(np.prod(it0["ne"])) * (it0["p"] + 1) ** 3
+    ramp = it0["vel"][0] * (1.0 + 0.1 * inputs.u01(77, np.arange(n0, dtype=np.uint64)))
+    for u0 in (ramp, np.full(n0, 0.25)):
+        with make_batch(form, ens) as b:
+            b.set_field(0, dgk.FIELD_U, u0)
+            b.step(4)
+            got = [b.get_field(k) for k in range(len(ens))]
+        for k, it in enumerate(ens):
+            with dgk.Solver.structured(form, *it["ne"], it["p"]) as s:
+                s.init_gaussian(it["centre"], it["sigma"], it["vel"])
+                if k == 0:
+                    s.set_field(dgk.FIELD_U, u0)
+                s.step(it["dt"], 4)
+                alone = s.get_field(dgk.FIELD_Q)
+            assert np.array_equal(got[k], alone), (k, u0[0] == u0[-1])
