@@ -258,7 +258,7 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  *           1 the VALU kernel with v, w in registers (half the LDS);
  *           candidates 2.. are the FP64 MFMA kernel's shapes as for HIP_MP,
  *           limited to 160 KiB of LDS; above p = 15 only the MFMA shapes.
- *           Defaults: the register VALU kernel at p = 3 and 6..13, the
+ *           Defaults: the register VALU kernel at p = 3, 4 and 6..13, the
  *           direction-by-direction MFMA shape (the last candidate) at
  *           p = 14, 15, the staged VALU kernel otherwise.
  * dgk_tile_select sets the process-wide choice for that form and order

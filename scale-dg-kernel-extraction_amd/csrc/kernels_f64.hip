@@ -84,6 +84,12 @@ struct Staged {
   // register form at p = 11, which otherwise lands one VGPR above the
   // 168 that 3 waves allow (and drops to 2).
   static constexpr int MIN_WAVES_EU = (VREG && NQ == 12) ? 3 : 1;
+  // The axis-aligned register form also at p = 3, 4 (5 waves: 100 and 118
+  // VGPRs -> 83-92, no spills; launches 6% faster) and p = 13 (3 waves: 183
+  // VGPRs -> 168 with 28 bytes per lane spilled; 4% faster).  The staged
+  // form ran slower held the same way.
+  static constexpr int MIN_WAVES_EU_AX = !VREG ? 1 : NQ == 12 ? 3 : (NQ == 4 || NQ == 5) ? 5
+                                         : NQ == 14 ? 3 : 1;
 };
 
 // AXAL: every face normal lies along its face's axis (checked on the host), so
@@ -99,7 +105,8 @@ struct Staged {
 // they equal, so the results are the same bits.
 template <int NQ, bool VREG, bool AXAL, bool CVEL>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, VREG>::THREADS),
-                               amdgpu_waves_per_eu(Staged<NQ, VREG>::MIN_WAVES_EU)))
+                               amdgpu_waves_per_eu(AXAL ? Staged<NQ, VREG>::MIN_WAVES_EU_AX
+                                                        : Staged<NQ, VREG>::MIN_WAVES_EU)))
 stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
           const double *__restrict__ v_, const double *__restrict__ w_, double *out_,
           const double *__restrict__ flux_in_, const int32_t *__restrict__ nbr_,
@@ -681,14 +688,14 @@ unsigned grid1d(int64_t n, int threads) {
 // kernels (stage_f64 / stage_f64_lift).
 constexpr int kValuShapes = 2;
 // Default per Nq, from whole-step timings of every candidate on MI355X at
-// about 1.7e7 DOF: the register VALU kernel at p = 3 and 6..13, the
+// about 1.7e7 DOF: the register VALU kernel at p = 3, 4 and 6..13, the
 // direction-by-direction MFMA kernel (the last candidate) at p = 14, 15, the
 // staged VALU kernel else.
-constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
+constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
                                          0, 0, 0, 0};
 // Selected FP64 tile shape per Nq (see stage_f64_shapes); atomic because a
 // serving worker may launch while another thread selects.
-std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1,
+std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1,
                                             3, 3, 0, 0, 0, 0};
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
