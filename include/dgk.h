@@ -201,8 +201,12 @@ int dgk_minmax_noalloc(const double *q, int64_t n, double *result, void *stream)
  * seconds[2] Element boundary flux (device, dgk_cal_elembnd_flux),
  * seconds[3] Volume derivate + surface lift (device, dgk_cal_dqdt),
  * seconds[4] fused stage kernels (device, dgk_step),
- * counts[0..4] number of timed intervals.  Device times are collected only
- * while dgk_set_kernel_timing(s, 1) is on (HIP events around each launch). */
+ * counts[0..3] number of timed intervals, counts[4] number of stage launches.
+ * Device times are collected only while dgk_set_kernel_timing(s, 1) is on:
+ * HIP events around each tendency launch, and one event pair around all
+ * 3 nstep stage launches of a dgk_step call (nothing is recorded between
+ * them, so seconds[4] / counts[4] is the mean launch duration of a
+ * back-to-back sequence, the gaps between launches included). */
 int dgk_set_kernel_timing(dgk_solver *s, int on);
 int dgk_timer_report(dgk_solver *s, double *seconds, int64_t *counts);
 int dgk_timer_reset(dgk_solver *s);

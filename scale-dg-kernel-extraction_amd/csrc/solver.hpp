@@ -57,7 +57,7 @@ struct dgk_solver {
   bool kernel_timing = false;
   double sec[dgk::T_N] = {0, 0, 0, 0, 0};
   int64_t cnt[dgk::T_N] = {0, 0, 0, 0, 0};
-  struct Pending { int slot; hipEvent_t a, b; };
+  struct Pending { int slot; hipEvent_t a, b; int64_t count; };  // count: launches spanned
   std::vector<Pending> pending;
   std::vector<hipEvent_t> free_events;
 };

@@ -157,17 +157,21 @@ def tile_shapes(evaltype, p: int) -> list:
     return e[:n.value].tolist()
 
 
-def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True) -> str:
-    """Name of the stage kernel that tile candidate `cand` launches (the FP64
-    VALU kernels come in an axis-aligned and a general face-normal form; the
-    structured meshes are axis-aligned)."""
+def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True,
+                constant_velocity: bool = False) -> str:
+    """Name of the stage kernel that tile candidate `cand` launches, as a
+    kernel trace prints it (the FP64 VALU kernels come in an axis-aligned and
+    a general face-normal form; the structured meshes are axis-aligned; the
+    last template argument selects the form that reads one constant velocity
+    from the geometry record, which batches of constant-velocity instances
+    run)."""
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
     nq = p + 1
     nvalu = 2 if evaltype == EVAL_HIP and nq <= 16 else 0  # FP64 VALU kernels first
     if cand < nvalu:
         b = lambda x: "true" if x else "false"
-        return f"stage_f64<{nq}, {b(cand == 1)}, {b(axis_aligned)}>"
+        return f"stage_f64<{nq}, {b(cand == 1)}, {b(axis_aligned)}, {b(constant_velocity)}>"
     shapes = tile_shapes(evaltype, p)
     # the last candidate is the kernel over runs of elements: FP64 at p <= 7,
     # mixed at p <= 4
@@ -187,7 +191,8 @@ def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True) -> str:
     one_wave = epb * nq * nq <= 64
     ax_runtime = seq or (evaltype != EVAL_HIP and one_wave)
     b = lambda x: "true" if x else "false"
-    return f"stage_mfma<{prec}, {nq}, {epb}, {b(seq)}, {b(axis_aligned and not ax_runtime)}>"
+    return (f"stage_mfma<{prec}, {nq}, {epb}, {b(seq)}, {b(axis_aligned and not ax_runtime)}, "
+            f"{b(constant_velocity)}>")
 
 
 def tile_select(evaltype, p: int, cand: int) -> None:
