@@ -80,6 +80,8 @@ def parse():
     ap.add_argument("--batch-only", action="store_true",
                     help="only build the 512-instance ensemble in --form and run --steps "
                          "graph-replayed steps (for kernel traces of a batched step)")
+    ap.add_argument("--per-order-table", type=int, nargs="*", default=None, metavar="P",
+                    help="only the per-order table, at these orders (all of 1..19 if none given)")
     ap.add_argument("--batch-orders", type=int, nargs="*", default=None, metavar="P",
                     help="with --batch-only: keep only the ensemble's instances of these orders")
     return ap.parse_args()
@@ -348,8 +350,11 @@ def per_order_only(dgk, form, p, steps):
                       "kernel": dgk.kernel_name(form, p, dgk.tile_selected(form, p))}), flush=True)
 
 
-def per_order_leg(dgk, orders=(1, 3, 5, 7, 9, 11, 13, 15), steps=20):
-    """Stage-kernel rate of both forms at every other order on a periodic box
+PER_ORDER_ORDERS = tuple(range(1, 20))
+
+
+def per_order_leg(dgk, orders=PER_ORDER_ORDERS, steps=30, warmup=30):
+    """Stage-kernel rate of both forms at every order 1..19 on a periodic box
     of about 1.68e7 DOF (Gaussian blob, constant velocity), default tile
     shapes: achieved algorithmic HBM rate and useful FLOP rate against the
     MI355X peaks (HBM 8 TB/s; MFMA FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s)."""
@@ -361,7 +366,10 @@ def per_order_leg(dgk, orders=(1, 3, 5, 7, 9, 11, 13, 15), steps=20):
             with dgk.Solver.structured(form, ne, ne, ne, p) as s:
                 s.init_gaussian((0.3, 0.6, 0.2), 0.1, (0.9, -0.4, 0.3))
                 dt = 0.05 * (2.0 / ne) / np_ ** (1.0 / 3.0)
-                s.step(dt, 2)
+                # untimed steps first: a launch sequence of a few milliseconds
+                # right after set-up runs below the rate the same kernel holds
+                # once the device has been busy for a while
+                s.step(dt, warmup)
                 s.synchronize()
                 s.timer_reset()
                 s.set_kernel_timing(True)
@@ -470,6 +478,10 @@ def main():
         return
     if args.per_order_only:
         per_order_only(dgk, args.form, args.per_order_only, args.steps)
+        return
+    if args.per_order_table is not None:
+        for row in per_order_leg(dgk, tuple(args.per_order_table) or PER_ORDER_ORDERS):
+            print(json.dumps(row), flush=True)
         return
     from dgk import inputs
 

@@ -91,7 +91,13 @@ struct MP {
   static constexpr int EPB = EPB_;
   static constexpr bool SEQ = SEQ_;
   static constexpr int LANES = EPB * NQ2;                   // node columns per block
-  static constexpr int THREADS = ((LANES + 63) / 64) * 64;
+  // Above p = 15 (289..400 columns) the block is eight wavefronts, two per
+  // SIMD, the idle lanes taking GEMM tiles, and held to 4 waves per SIMD, so
+  // two blocks share a CU (per launch at about 1.7e7 DOF, p = 16..19, against
+  // 5..7 wavefronts and no cap: FP64 314/276/308/349 -> 278/270/263/320 us,
+  // mixed 171/153/180/209 -> 151/166/176/193 us)
+  static constexpr bool HI = SEQ && NQ > 16;
+  static constexpr int THREADS = HI ? 512 : ((LANES + 63) / 64) * 64;
   static constexpr int WAVES = THREADS / 64;
   static constexpr int ROWS = ((LANES + 15) / 16) * 16;     // pencils per direction
   static constexpr int MTILES = ROWS / 16;
@@ -118,8 +124,9 @@ struct MP {
       XD ? (size_t)EPB * 4 * NQ2 * (sizeof(double) + sizeof(typename P::vel_t)) : 0;
   static constexpr size_t LDS_BYTES = (A_BYTES > XCH_BYTES ? A_BYTES : XCH_BYTES) + XD_BYTES;
   // Register budget: the mixed exchange shapes keep 4 waves per SIMD (at most
-  // 128 registers; left free, p = 7 lands two above and drops to 3)
-  static constexpr int MIN_WAVES_EU = (sizeof(typename P::op_t) == 4 && XCH) ? 4 : 1;
+  // 128 registers; left free, p = 7 lands two above and drops to 3), and so
+  // do the shapes above p = 15 (FP64 spills 12..60 bytes per lane)
+  static constexpr int MIN_WAVES_EU = ((sizeof(typename P::op_t) == 4 && XCH) || HI) ? 4 : 1;
   // Shapes whose face-normal form is tested at run time (see stage_mfma).
   static constexpr bool AX_RUNTIME = SEQ || MIN_WAVES_EU > 1;
   static constexpr int JOBS = NAREA * MTILES;               // (direction, tile) GEMMs per pass
@@ -508,11 +515,6 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
 #pragma unroll
         for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
       }
-      if (d == 2 && active && use_q0) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k)
-        q0v[k] = NT_SLAB ? __builtin_nontemporal_load(q0_ + base + k * NQ2) : q0_[base + k * NQ2];
-      }
       __syncthreads();
 #pragma unroll
       for (int jj = 0; jj < C::JPW; ++jj) {
@@ -533,6 +535,14 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   }
 
   // ---- 4b. RK epilogue (FP64) -------------------------------------------------
+  // SEQ reads the RK base only here: held through the last direction's GEMM it
+  // cost p = 16, 17 a wave per SIMD (FP64 Nq = 17: 154 -> 124 VGPRs; p = 16
+  // 399 -> 313 us per launch, p = 17 344 -> 276, mixed p = 19 271 -> 208)
+  if (SEQ && active && use_q0) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k)
+      q0v[k] = NT_SLAB ? __builtin_nontemporal_load(q0_ + base + k * NQ2) : q0_[base + k * NQ2];
+  }
   if (active) {
     const double dtv = dt_geo_ ? g[kGeoDt] : dt_;
 #pragma unroll
