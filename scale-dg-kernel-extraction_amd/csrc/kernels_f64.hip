@@ -103,7 +103,9 @@ struct Staged {
 // of its geometry record (batched and served boxes); no velocity field is
 // read.  The constants enter the same expressions as the per-node values
 // they equal, so the results are the same bits.
-template <int NQ, bool VREG, bool AXAL, bool CVEL>
+// GU: several elements per workgroup (p <= 4) and one geometry record for the
+// whole launch (no per-element record ids).
+template <int NQ, bool VREG, bool AXAL, bool CVEL, bool GU>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, VREG>::THREADS),
                                amdgpu_waves_per_eu(AXAL ? Staged<NQ, VREG>::MIN_WAVES_EU_AX
                                                         : Staged<NQ, VREG>::MIN_WAVES_EU)))
@@ -137,17 +139,26 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   const int64_t slot = slot0 + el;
   const bool active = lane_ok && slot < Ne_;
   int64_t e;
-  const double *g;
+  const double *g_lane;
   if (EPB == 1) {
     // one element per workgroup: its index, neighbour table entries and
     // geometry record are wave-uniform and read through the scalar cache
     e = slot < Ne_ ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
     e = uniform(e);
-    g = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+    g_lane = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   } else {
     e = active ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
-    g = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+    g_lane = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   }
+  // The rest runs with g = the element's geometry record.  With several
+  // elements per workgroup and one record for the whole launch (GU) it is
+  // entered with the kernel argument itself, so the record's reads are
+  // scalar-cache loads: as per-lane loads they were a memory round trip of
+  // their own after the field loads had landed (3.7-4.3 us of a 12-16 us
+  // workgroup life at p = 1..4, from in-kernel timestamps; launches 7-9%
+  // faster without it).  A compile-time choice, so each form holds only the
+  // registers of its own path.
+  auto body = [&](const double *__restrict__ g) {
   const int64_t base = e * NP + ij;
   const int lb = el * NP;  // element block in the staged fields
   const bool rk = mode_ == 1;
@@ -359,6 +370,9 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
       else out_[idx] = res;
     }
   }
+  };
+  if constexpr (GU) body(geo_);
+  else body(g_lane);
 }
 
 // Stored-Lift kernel: Lift_mat read as stored (any Lift_mat).
@@ -496,18 +510,31 @@ stage_f64_lift(const StageArgs P) {
   }
 }
 
-template <int NQ, bool VREG, bool AXAL, bool CVEL = false>
-int prepare_shape_valu() {
+template <int NQ, bool VREG, bool AXAL, bool CVEL, bool GU>
+int prepare_form_valu() {
   // set once per device; concurrent callers may both set it (idempotent)
   static std::atomic<int> prepared_dev{-1};
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev.load(std::memory_order_acquire) != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG, AXAL, CVEL>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64<NQ, VREG, AXAL, CVEL, GU>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)Staged<NQ, VREG>::LDS_BYTES));
     prepared_dev.store(dev, std::memory_order_release);
   }
+  return DGK_OK;
+}
+
+// the uniform-geometry form exists where a workgroup holds several elements
+template <int NQ, bool VREG>
+constexpr bool has_gu() {
+  return Staged<NQ, VREG>::EPB > 1;
+}
+
+template <int NQ, bool VREG, bool AXAL, bool CVEL = false>
+int prepare_shape_valu() {
+  DGK_TRY((prepare_form_valu<NQ, VREG, AXAL, CVEL, false>()));
+  if constexpr (has_gu<NQ, VREG>()) DGK_TRY((prepare_form_valu<NQ, VREG, AXAL, CVEL, true>()));
   return DGK_OK;
 }
 
@@ -521,18 +548,25 @@ int prepare_nq() {
   return prepare_shape_valu<NQ, true, true>();
 }
 
-template <int NQ, bool VREG, bool AXAL, bool CVEL = false>
-int launch_valu(const StageArgs &a, hipStream_t s) {
+template <int NQ, bool VREG, bool AXAL, bool CVEL, bool GU>
+int launch_form_valu(const StageArgs &a, hipStream_t s) {
   using T = Staged<NQ, VREG>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  DGK_TRY((prepare_shape_valu<NQ, VREG, AXAL, CVEL>()));
-  hipLaunchKernelGGL((stage_f64<NQ, VREG, AXAL, CVEL>), dim3((unsigned)nblk), dim3(T::THREADS),
-                     T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.flux_in, a.nbr,
-                     a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt,
-                     a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
+  DGK_TRY((prepare_form_valu<NQ, VREG, AXAL, CVEL, GU>()));
+  hipLaunchKernelGGL((stage_f64<NQ, VREG, AXAL, CVEL, GU>), dim3((unsigned)nblk),
+                     dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out,
+                     a.flux_in, a.nbr, a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca,
+                     a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
+}
+
+template <int NQ, bool VREG, bool AXAL, bool CVEL = false>
+int launch_valu(const StageArgs &a, hipStream_t s) {
+  if constexpr (has_gu<NQ, VREG>())
+    if (!a.geoid) return launch_form_valu<NQ, VREG, AXAL, CVEL, true>(a, s);
+  return launch_form_valu<NQ, VREG, AXAL, CVEL, false>(a, s);
 }
 
 // VALU stage kernel (VREG: v, w in registers; the axis-aligned or general

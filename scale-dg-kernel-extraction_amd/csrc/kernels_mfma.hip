@@ -217,7 +217,10 @@ __device__ __forceinline__ void mfma_tile(typename P::op_t *tile,
 // geometry record (batched and served boxes); no velocity field is read, and
 // the operand-precision value of a constant is the value its per-node copy
 // holds, so the results are the same bits.
-template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL = false>
+// GU: several elements per workgroup and one geometry record for the whole
+// launch (no per-element record ids): the record is read through the kernel
+// argument itself, i.e. through the scalar cache (as in stage_f64).
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL = false, bool GU = false>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, MP<P, NQ, EPB, SEQ>::THREADS),
                                amdgpu_waves_per_eu(MP<P, NQ, EPB, SEQ>::MIN_WAVES_EU)))
 stage_mfma(const double *__restrict__ q_, const double *q0_,
@@ -247,15 +250,17 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   const int64_t slot = slot0 + el;
   const bool active = lane_ok && slot < Ne_;
   int64_t e;
-  const double *g;
+  const double *g_lane;
   if (EPB == 1) {
     e = slot < Ne_ ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
     e = uniform_mp(e);
-    g = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+    g_lane = geo_ + (geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   } else {
     e = active ? slot_element(slot, Ne_, order_, rev_, zp_) : 0;
-    g = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
+    g_lane = geo_ + (active && geoid_ ? (int64_t)geoid_[e] * kGeoStride : 0);
   }
+  static_assert(!GU || EPB > 1, "one element per workgroup reads its record uniformly anyway");
+  auto body = [&](const double *__restrict__ g) {
   const int64_t ebase = e * NP;
   const int64_t base = ebase + ij;
   const bool rk = mode_ == 1;
@@ -558,6 +563,9 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
       else out_[idx] = res;
     }
   }
+  };
+  if constexpr (GU) body(geo_);
+  else body(g_lane);
 }
 
 // ---- mixed precision over runs of consecutive elements (p <= 4) -----------
@@ -888,18 +896,27 @@ int launch_run_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   }
 }
 
-template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL = false>
-int prepare_form() {
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL, bool GU>
+int prepare_kernel() {
   // set once per device; concurrent callers may both set it (idempotent)
   static std::atomic<int> prepared_dev{-1};
   int dev = 0;
   DGK_HIP(hipGetDevice(&dev));
   if (prepared_dev.load(std::memory_order_acquire) != dev) {
-    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ, AXAL, CVEL>,
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_mfma<P, NQ, EPB, SEQ, AXAL, CVEL, GU>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)MP<P, NQ, EPB, SEQ>::LDS_BYTES));
     prepared_dev.store(dev, std::memory_order_release);
   }
+  return DGK_OK;
+}
+
+// a form with its uniform-geometry variant where a workgroup holds several
+// elements
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL = false>
+int prepare_form() {
+  DGK_TRY((prepare_kernel<P, NQ, EPB, SEQ, AXAL, CVEL, false>()));
+  if constexpr (EPB > 1) DGK_TRY((prepare_kernel<P, NQ, EPB, SEQ, AXAL, CVEL, true>()));
   return DGK_OK;
 }
 
@@ -914,18 +931,25 @@ int prepare_shape() {
   return DGK_OK;
 }
 
-template <class P, int NQ, int EPB, bool SEQ, bool AXAL, class Args, bool CVEL = false>
-int launch_form(const Args &a, hipStream_t s) {
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, bool CVEL, bool GU, class Args>
+int launch_kernel(const Args &a, hipStream_t s) {
   using T = MP<P, NQ, EPB, SEQ>;
   const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
   if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  DGK_TRY((prepare_form<P, NQ, EPB, SEQ, AXAL, CVEL>()));
-  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ, AXAL, CVEL>), dim3((unsigned)nblk),
+  DGK_TRY((prepare_kernel<P, NQ, EPB, SEQ, AXAL, CVEL, GU>()));
+  hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ, AXAL, CVEL, GU>), dim3((unsigned)nblk),
                      dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr,
                      a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt,
                      a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
+}
+
+template <class P, int NQ, int EPB, bool SEQ, bool AXAL, class Args, bool CVEL = false>
+int launch_form(const Args &a, hipStream_t s) {
+  if constexpr (EPB > 1)
+    if (!a.geoid) return launch_kernel<P, NQ, EPB, SEQ, AXAL, CVEL, true>(a, s);
+  return launch_kernel<P, NQ, EPB, SEQ, AXAL, CVEL, false>(a, s);
 }
 
 template <class P, int NQ, int EPB, bool SEQ, class Args>

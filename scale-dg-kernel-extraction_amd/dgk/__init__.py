@@ -158,20 +158,24 @@ def tile_shapes(evaltype, p: int) -> list:
 
 
 def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True,
-                constant_velocity: bool = False) -> str:
+                constant_velocity: bool = False, uniform_geometry: bool = True) -> str:
     """Name of the stage kernel that tile candidate `cand` launches, as a
-    kernel trace prints it (the FP64 VALU kernels come in an axis-aligned and
-    a general face-normal form; the structured meshes are axis-aligned; the
-    last template argument selects the form that reads one constant velocity
-    from the geometry record, which batches of constant-velocity instances
-    run)."""
+    kernel trace prints it.  The FP64 VALU kernels come in an axis-aligned and
+    a general face-normal form (the structured meshes are axis-aligned); the
+    constant-velocity form reads one velocity from the geometry record
+    (batches of constant-velocity instances, served requests); the last
+    template argument marks a workgroup of several elements reading one
+    geometry record for the whole launch (single meshes and served requests,
+    not batches, whose instances have records of their own)."""
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
     nq = p + 1
     nvalu = 2 if evaltype == EVAL_HIP and nq <= 16 else 0  # FP64 VALU kernels first
     if cand < nvalu:
         b = lambda x: "true" if x else "false"
-        return f"stage_f64<{nq}, {b(cand == 1)}, {b(axis_aligned)}, {b(constant_velocity)}>"
+        gu = uniform_geometry and nq * nq <= 32  # several elements per workgroup
+        return (f"stage_f64<{nq}, {b(cand == 1)}, {b(axis_aligned)}, {b(constant_velocity)}, "
+                f"{b(gu)}>")
     shapes = tile_shapes(evaltype, p)
     # the last candidate is the kernel over runs of elements: FP64 at p <= 7,
     # mixed at p <= 4
@@ -192,7 +196,7 @@ def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True,
     ax_runtime = seq or (evaltype != EVAL_HIP and one_wave)
     b = lambda x: "true" if x else "false"
     return (f"stage_mfma<{prec}, {nq}, {epb}, {b(seq)}, {b(axis_aligned and not ax_runtime)}, "
-            f"{b(constant_velocity)}>")
+            f"{b(constant_velocity)}, {b(uniform_geometry and epb > 1)}>")
 
 
 def tile_select(evaltype, p: int, cand: int) -> None:

@@ -155,7 +155,7 @@ def test_tile_shapes_host_side():
     assert dgk.tile_shapes("HIP", 7) == [1, 1, 1, 2, 4, 1, 1]
     assert dgk.tile_shapes("HIP", 3) == [4, 4, 4, 8, 16, 4, 8]
     assert dgk.kernel_name("HIP", 7, 6) == "stage_run<8, true>"
-    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true, false, false>"
+    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true, false, false, false>"
     assert dgk.tile_shapes("HIP", 15) == [1, 1, 1, 1]
     with pytest.raises(dgk.DgkError) as ei:
         dgk.tile_select("HIP", 15, 4)
@@ -195,16 +195,22 @@ def test_shapes_above_the_operator_files():
         assert dgk.tile_shapes("HIP", p) == [1]
         assert dgk.tile_shapes("HIP_MP", p) == [1]
         assert dgk.tile_selected("HIP", p) == 0
-    assert dgk.kernel_name("HIP", 17, 0) == "stage_mfma<PrecF64, 18, 1, true, false, false>"
-    assert dgk.kernel_name("HIP", 7, 0) == "stage_f64<8, false, true, false>"
-    assert dgk.kernel_name("HIP", 7, 1) == "stage_f64<8, true, true, false>"
-    assert dgk.kernel_name("HIP", 7, 1, axis_aligned=False) == "stage_f64<8, true, false, false>"
-    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true, false, false>"
-    assert dgk.kernel_name("HIP", 7, 2) == "stage_mfma<PrecF64, 8, 1, false, true, false>"
-    assert dgk.kernel_name("HIP_MP", 7, 0) == "stage_mfma<PrecMixed, 8, 1, false, false, false>"
-    assert dgk.kernel_name("HIP_MP", 11, 0) == "stage_mfma<PrecMixed, 12, 1, false, true, false>"
-    assert dgk.kernel_name("HIP_MP", 15, 2) == "stage_mfma<PrecMixed, 16, 1, true, false, false>"
-    assert dgk.kernel_name("HIP", 7, 1, constant_velocity=True) == "stage_f64<8, true, true, true>"
+    assert dgk.kernel_name("HIP", 17, 0) == "stage_mfma<PrecF64, 18, 1, true, false, false, false>"
+    assert dgk.kernel_name("HIP", 7, 0) == "stage_f64<8, false, true, false, false>"
+    assert dgk.kernel_name("HIP", 7, 1) == "stage_f64<8, true, true, false, false>"
+    assert dgk.kernel_name("HIP", 7, 1, axis_aligned=False) == "stage_f64<8, true, false, false, false>"
+    assert dgk.kernel_name("HIP", 7, 5) == "stage_mfma<PrecF64, 8, 1, true, false, false, false>"
+    assert dgk.kernel_name("HIP", 7, 2) == "stage_mfma<PrecF64, 8, 1, false, true, false, false>"
+    assert dgk.kernel_name("HIP_MP", 7, 0) == "stage_mfma<PrecMixed, 8, 1, false, false, false, false>"
+    assert dgk.kernel_name("HIP_MP", 11, 0) == "stage_mfma<PrecMixed, 12, 1, false, true, false, false>"
+    assert dgk.kernel_name("HIP_MP", 15, 2) == "stage_mfma<PrecMixed, 16, 1, true, false, false, false>"
+    assert dgk.kernel_name("HIP", 7, 1, constant_velocity=True) == "stage_f64<8, true, true, true, false>"
     assert dgk.kernel_name("HIP_MP", 7, 0, constant_velocity=True) == (
-        "stage_mfma<PrecMixed, 8, 1, false, false, true>")
+        "stage_mfma<PrecMixed, 8, 1, false, false, true, false>")
+    # several elements per workgroup: the form reading one geometry record
+    # through the scalar cache, unless each element has a record of its own
+    assert dgk.kernel_name("HIP", 3, 1) == "stage_f64<4, true, true, false, true>"
+    assert dgk.kernel_name("HIP", 3, 1, uniform_geometry=False) == (
+        "stage_f64<4, true, true, false, false>")
+    assert dgk.kernel_name("HIP_MP", 3, 1) == "stage_mfma<PrecMixed, 4, 8, false, true, false, true>"
     assert dgk.element_sizes(19) == dict(Nq=20, Np=8000, Nfp=400, NfpTot=2400)
