@@ -731,14 +731,15 @@ unsigned grid1d(int64_t n, int threads) {
 // kernels (stage_f64 / stage_f64_lift).
 constexpr int kValuShapes = 2;
 // Default per Nq, from whole-step timings of every candidate on MI355X at
-// about 1.7e7 DOF: the register VALU kernel at p = 3, 4 and 6..13, the
+// about 1.7e7 DOF: the register VALU kernel at p = 4 and 6..13, the
 // direction-by-direction MFMA kernel (the last candidate) at p = 14, 15, the
-// staged VALU kernel else.
-constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
+// staged VALU kernel else (p = 3: 158 -> 153 us per launch once both read a
+// launch-wide geometry record through the scalar cache).
+constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
                                          0, 0, 0, 0};
 // Selected FP64 tile shape per Nq (see stage_f64_shapes); atomic because a
 // serving worker may launch while another thread selects.
-std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1,
+std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1,
                                             3, 3, 0, 0, 0, 0};
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {

@@ -973,13 +973,14 @@ struct Cand {
 
 // Tile-shape candidate selected per Nq for the mixed form.  Defaults from
 // per-order timings on MI355X at about 1.7e7 DOF (whole-step runs, every
-// candidate): the two-wavefront shape at p = 1..4 (its axis-aligned form
-// holds fewer registers than the one-wavefront shape, which keeps the
-// run-time test), the one-wavefront shape above, and the direction-by-
-// direction shape at p = 14.
-constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 1, 1, 1, 1, 0, 0, 0, 0, 0,
+// candidate, with a launch-wide geometry record read through the scalar
+// cache): the two-wavefront shape at p = 1, 4, the four-wavefront shape at
+// p = 5 (136 -> 126 us per launch), the direction-by-direction shape at
+// p = 14, the one-wavefront shape else (p = 2: 178 -> 174 us, p = 3:
+// 133 -> 121 us against two wavefronts).
+constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 1, 0, 0, 1, 2, 0, 0, 0, 0,
                                         0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
-std::atomic<int> g_mp_shape[kMaxNq + 1] = {0, 0, 1, 1, 1, 1, 0, 0, 0, 0, 0,
+std::atomic<int> g_mp_shape[kMaxNq + 1] = {0, 0, 1, 0, 0, 1, 2, 0, 0, 0, 0,
                                            0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
 
 template <class P, int NQ>

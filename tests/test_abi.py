@@ -170,19 +170,19 @@ def test_tile_shapes_host_side():
     assert dgk.tile_selected("HIP_MP", 7) == 2
     dgk.tile_select("HIP_MP", 7, -1)
     assert dgk.tile_selected("HIP_MP", 7) == 0
-    # mixed defaults: two wavefronts at p = 1..4, one above, direction by
-    # direction at p = 14
+    # mixed defaults: two wavefronts at p = 1, 4, four at p = 5, direction
+    # by direction at p = 14, one wavefront (or one element) else
     assert [dgk.tile_selected("HIP_MP", p) for p in range(1, 16)] == (
-        [1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
-    dgk.tile_select("HIP_MP", 2, 0)
-    assert dgk.tile_selected("HIP_MP", 2) == 0
-    dgk.tile_select("HIP_MP", 2, -1)
-    assert dgk.tile_selected("HIP_MP", 2) == 1
-    # FP64 defaults: the register VALU kernel at p = 3, 4 and 6..13, the
+        [1, 0, 0, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
+    dgk.tile_select("HIP_MP", 1, 0)
+    assert dgk.tile_selected("HIP_MP", 1) == 0
+    dgk.tile_select("HIP_MP", 1, -1)
+    assert dgk.tile_selected("HIP_MP", 1) == 1
+    # FP64 defaults: the register VALU kernel at p = 4 and 6..13, the
     # direction-by-direction MFMA kernel (the last candidate) at p = 14, 15,
     # the staged VALU kernel else
     assert [dgk.tile_selected("HIP", p) for p in range(1, 16)] == (
-        [0, 0, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3])
+        [0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3])
     dgk.tile_select("HIP", 13, 0)
     assert dgk.tile_selected("HIP", 13) == 0
     dgk.tile_select("HIP", 13, -1)
