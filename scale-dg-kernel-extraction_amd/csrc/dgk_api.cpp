@@ -257,9 +257,10 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(set_error(DGK_ERR_DEVICE, "hipStreamCreate failed"));
   for (int b = 0; b < 3; ++b)
-    if ((rc = dalloc(s, &s->q[b], s->nfield))) return fail(rc);
-  if ((rc = dalloc(s, &s->u, s->nfield)) || (rc = dalloc(s, &s->v, s->nfield)) ||
-      (rc = dalloc(s, &s->w, s->nfield)))
+    if ((rc = dalloc(s, &s->q[b], s->nfield + kStagePad))) return fail(rc);
+  if ((rc = dalloc(s, &s->u, s->nfield + kStagePad)) ||
+      (rc = dalloc(s, &s->v, s->nfield + kStagePad)) ||
+      (rc = dalloc(s, &s->w, s->nfield + kStagePad)))
     return fail(rc);
   if ((rc = dalloc(s, &s->nbr, (int64_t)t.nbr.size())) ||
       (rc = dalloc(s, &s->vmapP, (int64_t)t.vmapP.size())) ||

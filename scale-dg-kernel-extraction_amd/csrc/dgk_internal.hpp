@@ -36,6 +36,10 @@ constexpr int kMinNq = 2;
 constexpr int kMaxNq = 20;      // p <= 19: operators above p = 15 are generated
 constexpr int kMaxNqFile = 16;  // operator_data/p{p}.dat exist for p <= 15
 constexpr int kMaxNqValu = 16;  // the VALU FP64 kernels; above, MFMA kernels only
+// Doubles of readable memory after the last node of every field the FP64 VALU
+// stage kernel stages to LDS (q, u, v, w): with an odd node count per element
+// its 16-byte copy granules end 8 bytes past an even-numbered element.
+constexpr int kStagePad = 2;
 
 // Per-element geometry record: Escale[3] (2/dx_d, constant per element,
 // mod_mesh.f90:147-149), Fscale[6] and the outward normal of each face

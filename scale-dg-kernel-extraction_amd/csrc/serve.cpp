@@ -127,7 +127,8 @@ Layout layout(int64_t Ne, int Np, bool mp) {
   // q in three rotation buffers; the velocity is one constant per request,
   // read by the stage kernels from the geometry record, so no velocity fields
   (void)mp;
-  for (int b = 0; b < 3; ++b) L.q[b] = take(dof * 8);
+  // (each with the tail pad the staged reads of the FP64 kernel may touch)
+  for (int b = 0; b < 3; ++b) L.q[b] = take((dof + kStagePad) * 8);
   L.u = L.v = L.w = -1;
   L.nbr = take(Ne * 6 * 4);
   L.geo = take(kGeoStride * 8);
