@@ -32,6 +32,7 @@ The JSON line also carries
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import os
 import sys
@@ -156,6 +157,20 @@ def traffic_from_profile(kernel):
             continue
         if d.get("kernel") == kernel:
             return d.get("bytes_per_launch")
+    return None
+
+
+def kernel_trace_from_profile(kernel):
+    """Mean duration (us) of `kernel` in the rocprofv3 --kernel-trace --stats
+    summaries of this workload in profiles/ (None when none covers it)."""
+    for f in sorted((ROOT / "profiles").glob("stage_*_c2_32cube_p7_kernel_stats.txt")):
+        try:
+            lines = f.read_text().splitlines()
+        except OSError:
+            continue
+        for row in csv.reader(l for l in lines if not l.startswith("#")):
+            if len(row) > 3 and (kernel + "(") in row[0].replace("dgk::(anonymous namespace)::", ""):
+                return float(row[3]) / 1e3
     return None
 
 
@@ -510,6 +525,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(kname), "kernel": kname,
+                     "timing": "one HIP event pair on the solver's stream around the timed "
+                               "dgk_step call's back-to-back stage launches",
+                     "kernel_trace_avg_us": kernel_trace_from_profile(kname),
                      "avg_launch_us": 1e6 * r["launch_s"], "launches": r["launches"],
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "autotune_us_per_launch": r["tune"]["us_per_launch"]},
