@@ -40,8 +40,9 @@ hipEvent_t take_event(dgk_solver *s) {
   return e;
 }
 
-// Record an event pair around a launch sequence when timing is on.
-// One event pair around `count` launches on the solver's stream.
+// One event pair on the solver's stream around a sequence of `count`
+// launches, when timing is on.  A sequence left before done() (an error
+// return) gives its events back to the pool.
 struct Timed {
   dgk_solver *s;
   int slot;
@@ -53,6 +54,12 @@ struct Timed {
     a = take_event(s);
     b = take_event(s);
     if (a && b) (void)hipEventRecord(a, s->stream);
+  }
+  Timed(const Timed &) = delete;
+  Timed &operator=(const Timed &) = delete;
+  ~Timed() {
+    if (a) s->free_events.push_back(a);
+    if (b) s->free_events.push_back(b);
   }
   void done() {
     if (a && b) {
