@@ -165,221 +165,221 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   // faster without it).  A compile-time choice, so each form holds only the
   // registers of its own path.
   auto body = [&](const double *__restrict__ g) {
-  const int64_t base = e * NP + ij;
-  // element block in the staged fields
-  const int lb = el * C::NPP + (C::ODD ? (int)(e & 1) : 0);
-  const bool rk = mode_ == 1;
-  const bool use_q0 = rk && ca_ != 0.0;
-  double cvel[3] = {0.0, 0.0, 0.0};
-  if constexpr (CVEL) {
-    cvel[0] = g[kGeoVel];
-    cvel[1] = g[kGeoVel + 1];
-    cvel[2] = g[kGeoVel + 2];
-  }
-
-  // With several elements per workgroup the neighbour-table entries are
-  // per-lane loads: issued first, so waiting for them before the trace
-  // gathers does not also wait for the async copies behind them.  With one
-  // element they are scalar loads, read where used (loading them ahead costs
-  // a register, which at p = 11 drops the kernel from 3 to 2 waves per SIMD).
-  const bool resolved = vmapP_ != nullptr;
-  int32_t nb6[6];
-  if constexpr (EPB > 1) {
-    if (active && !flux_in_) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
-  }
-
-  // ---- 1. async HBM -> LDS copy of q, u, v, w -------------------------------
-  {
-    const int wave = tid >> 6, lane = tid & 63;
-    const double *src[4] = {q_, u_, v_, w_};
-#pragma unroll
-    for (int c0 = 0; c0 < C::NCHUNK; c0 += C::WAVES) {
-      const int c = c0 + wave;
-      if (c < C::NCHUNK) {
-        const int d = c * C::CHUNK + 2 * lane;  // first slot double of the granule
-        const int del = d / C::NPP, dn = d - del * C::NPP;
-        const int64_t sl = slot0 + del;
-        int64_t off = 0;
-        if (del < EPB && sl < Ne_) {
-          const int64_t ge = slot_element(sl, Ne_, order_, rev_, zp_);
-          off = ge * NP + dn - (C::ODD ? (ge & 1) : 0);
-        }
-#pragma unroll
-        for (int fld = 0; fld < (CVEL ? 1 : C::NSTAGED); ++fld) {
-          const double *gp = src[fld] + off;
-          __attribute__((address_space(3))) void *lp =
-              (__attribute__((address_space(3))) void *)(smem + fld * C::FIELD + c * C::CHUNK);
-#if defined(__HIP_DEVICE_COMPILE__)
-          __builtin_amdgcn_global_load_lds((const void *)gp, lp, 16, 0, 0);
-#else
-          (void)gp;
-          (void)lp;
-#endif
-        }
-      }
+    const int64_t base = e * NP + ij;
+    // element block in the staged fields
+    const int lb = el * C::NPP + (C::ODD ? (int)(e & 1) : 0);
+    const bool rk = mode_ == 1;
+    const bool use_q0 = rk && ca_ != 0.0;
+    double cvel[3] = {0.0, 0.0, 0.0};
+    if constexpr (CVEL) {
+      cvel[0] = g[kGeoVel];
+      cvel[1] = g[kGeoVel + 1];
+      cvel[2] = g[kGeoVel + 2];
     }
-  }
 
-  // neighbour traces of face node (i,j) on all six faces, and the RK base
-  // column, in flight together with the async copy (axis-aligned: q and the
-  // normal velocity component, fixed per face at compile time)
-  constexpr bool AX_RUNTIME = VREG && NQ == 12;
-  const bool axal = AX_RUNTIME ? axis_aligned_ != 0 : AXAL;
-  double qP[6], uP[6], vP[6], wP[6], q0v[NQ];
-  if (active && !flux_in_) {
-    auto nbr_at = [&](int f) -> int64_t {
-      if constexpr (EPB > 1)
-        return nbr_node<NQ>(resolved, nb6[f], f, i, j);
-      else
-        return resolved ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
-                        : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
-    };
-    if (axal) {
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        const int64_t pidx = nbr_at(f);
-        const double *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
-        qP[f] = q_[pidx];
-        uP[f] = CVEL ? cvel[kFaceAxis[f]] : vel[pidx];
-      }
-    } else {
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        const int64_t pidx = nbr_at(f);
-        qP[f] = q_[pidx];
-        uP[f] = u_[pidx];
-        vP[f] = v_[pidx];
-        wP[f] = w_[pidx];
-      }
+    // With several elements per workgroup the neighbour-table entries are
+    // per-lane loads: issued first, so waiting for them before the trace
+    // gathers does not also wait for the async copies behind them.  With one
+    // element they are scalar loads, read where used (loading them ahead costs
+    // a register, which at p = 11 drops the kernel from 3 to 2 waves per SIMD).
+    const bool resolved = vmapP_ != nullptr;
+    int32_t nb6[6];
+    if constexpr (EPB > 1) {
+      if (active && !flux_in_) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
     }
-  }
-  // VREG: the v and w columns and the v traces of the y faces
-  double vc[VREG ? NQ : 1], wc[VREG ? NQ : 1], vM[2];
-  if (VREG && active) {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      vc[k] = CVEL ? cvel[1] : v_[base + (int64_t)k * NQ2];
-      wc[k] = CVEL ? cvel[2] : w_[base + (int64_t)k * NQ2];
-    }
-    vM[0] = CVEL ? cvel[1] : v_[e * NP + own_face_node(0, i, j, NQ)];
-    vM[1] = CVEL ? cvel[1] : v_[e * NP + own_face_node(2, i, j, NQ)];
-  }
-  // the RK base and the output are touched once per launch: non-temporal,
-  // so L2 keeps the neighbour traces -- where a k-slab of an element is a
-  // run of whole 128-byte lines.  At p <= 6 with other slab sizes one
-  // instruction covers pieces of lines, which non-temporal stores send to
-  // memory as partial writes (twice the output's write requests at p = 1, 2)
-  // and non-temporal loads fetch again per piece; cached accesses let the
-  // L2 merge them (FP64 p = 1, 2, 4: 12%, 16%, 8% faster per launch).
-  constexpr bool NT_SLAB = (NQ2 % 16 == 0) || NQ > 7;
-  if (active && use_q0) {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k)
-      q0v[k] = NT_SLAB ? __builtin_nontemporal_load(q0_ + base + (int64_t)k * NQ2)
-                  : q0_[base + (int64_t)k * NQ2];
-  }
-  // D and the 1D Lift go to LDS last: the wait for them comes after every
-  // other load is in flight
-  for (int t = tid; t < NQ2; t += C::THREADS) sD[t] = D_[t];
-  for (int t = tid; t < 2 * NQ; t += C::THREADS) sL[t] = ell_[t];
-  __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
-  __syncthreads();
 
-  // ---- 2. upwind flux of face node (i,j) on each face ----------------------
-  double gf[6];
-  if (active) {
+    // ---- 1. async HBM -> LDS copy of q, u, v, w -------------------------------
+    {
+      const int wave = tid >> 6, lane = tid & 63;
+      const double *src[4] = {q_, u_, v_, w_};
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-      if (flux_in_) {
-        gf[f] = flux_in_[e * NFT + f * NQ2 + ij];
-      } else {
-        const double n1 = g[kGeoNormal + 3 * f], n2 = g[kGeoNormal + 3 * f + 1],
-                     n3 = g[kGeoNormal + 3 * f + 2];
-        const int m = lb + own_face_node(f, i, j, NQ);
-        const double qM = sq[m];
-        double VelM, VelP;
-        if (axal) {
-          double vm;
-          if constexpr (CVEL)
-            vm = cvel[kFaceAxis[f]];
-          else if constexpr (VREG)
-            vm = kFaceAxis[f] == 0 ? su[m] : f == 0 ? vM[0] : f == 2 ? vM[1]
-                 : f == 4 ? wc[0] : wc[NQ - 1];
-          else
-            vm = (kFaceAxis[f] == 0 ? su : kFaceAxis[f] == 1 ? sv : sw)[m];
-          const double na = g[kGeoNormal + 3 * f + kFaceAxis[f]];
-          VelM = vm * na;
-          VelP = uP[f] * na;
-        } else {
-          if constexpr (VREG) {
-            const int64_t gm = e * NP + own_face_node(f, i, j, NQ);
-            VelM = su[m] * n1 + v_[gm] * n2 + w_[gm] * n3;
-          } else {
-            VelM = su[m] * n1 + sv[m] * n2 + sw[m] * n3;
+      for (int c0 = 0; c0 < C::NCHUNK; c0 += C::WAVES) {
+        const int c = c0 + wave;
+        if (c < C::NCHUNK) {
+          const int d = c * C::CHUNK + 2 * lane;  // first slot double of the granule
+          const int del = d / C::NPP, dn = d - del * C::NPP;
+          const int64_t sl = slot0 + del;
+          int64_t off = 0;
+          if (del < EPB && sl < Ne_) {
+            const int64_t ge = slot_element(sl, Ne_, order_, rev_, zp_);
+            off = ge * NP + dn - (C::ODD ? (ge & 1) : 0);
           }
-          VelP = uP[f] * n1 + vP[f] * n2 + wP[f] * n3;
+#pragma unroll
+          for (int fld = 0; fld < (CVEL ? 1 : C::NSTAGED); ++fld) {
+            const double *gp = src[fld] + off;
+            __attribute__((address_space(3))) void *lp =
+                (__attribute__((address_space(3))) void *)(smem + fld * C::FIELD + c * C::CHUNK);
+#if defined(__HIP_DEVICE_COMPILE__)
+            __builtin_amdgcn_global_load_lds((const void *)gp, lp, 16, 0, 0);
+#else
+            (void)gp;
+            (void)lp;
+#endif
+          }
         }
-        const double alpha = 0.5 * fabs(VelP + VelM);
-        gf[f] = 0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qM * VelM - alpha * (qP[f] - qM));
       }
     }
-#pragma unroll
-    for (int f = 0; f < 4; ++f) sG[(el * 4 + f) * NQ2 + ij] = gf[f];
-  }
-  __syncthreads();
 
-  // ---- 3. volume fluxes: Fx, Fy in LDS, Fz and q in registers ---------------
-  double qc[NQ], fz[NQ];
-  if (active) {
+    // neighbour traces of face node (i,j) on all six faces, and the RK base
+    // column, in flight together with the async copy (axis-aligned: q and the
+    // normal velocity component, fixed per face at compile time)
+    constexpr bool AX_RUNTIME = VREG && NQ == 12;
+    const bool axal = AX_RUNTIME ? axis_aligned_ != 0 : AXAL;
+    double qP[6], uP[6], vP[6], wP[6], q0v[NQ];
+    if (active && !flux_in_) {
+      auto nbr_at = [&](int f) -> int64_t {
+        if constexpr (EPB > 1)
+          return nbr_node<NQ>(resolved, nb6[f], f, i, j);
+        else
+          return resolved ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
+                          : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+      };
+      if (axal) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int n = lb + ij + k * NQ2;
-      const double qq = sq[n], uu = CVEL ? cvel[0] : su[n];
-      const double vv = CVEL ? cvel[1] : VREG ? vc[k] : sv[n];
-      const double ww = CVEL ? cvel[2] : VREG ? wc[k] : sw[n];
-      qc[k] = qq;
-      fz[k] = qq * ww;
-      if (C::FX_T) su[lb + j + i * NQ + k * NQ2] = qq * uu;
-      else su[n] = qq * uu;
-      (VREG ? sq : sv)[n] = qq * vv;  // Fy (in place of q with VREG)
+        for (int f = 0; f < 6; ++f) {
+          const int64_t pidx = nbr_at(f);
+          const double *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
+          qP[f] = q_[pidx];
+          uP[f] = CVEL ? cvel[kFaceAxis[f]] : vel[pidx];
+        }
+      } else {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          const int64_t pidx = nbr_at(f);
+          qP[f] = q_[pidx];
+          uP[f] = u_[pidx];
+          vP[f] = v_[pidx];
+          wP[f] = w_[pidx];
+        }
+      }
     }
-  }
-  __syncthreads();
+    // VREG: the v and w columns and the v traces of the y faces
+    double vc[VREG ? NQ : 1], wc[VREG ? NQ : 1], vM[2];
+    if (VREG && active) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        vc[k] = CVEL ? cvel[1] : v_[base + (int64_t)k * NQ2];
+        wc[k] = CVEL ? cvel[2] : w_[base + (int64_t)k * NQ2];
+      }
+      vM[0] = CVEL ? cvel[1] : v_[e * NP + own_face_node(0, i, j, NQ)];
+      vM[1] = CVEL ? cvel[1] : v_[e * NP + own_face_node(2, i, j, NQ)];
+    }
+    // the RK base and the output are touched once per launch: non-temporal,
+    // so L2 keeps the neighbour traces -- where a k-slab of an element is a
+    // run of whole 128-byte lines.  At p <= 6 with other slab sizes one
+    // instruction covers pieces of lines, which non-temporal stores send to
+    // memory as partial writes (twice the output's write requests at p = 1, 2)
+    // and non-temporal loads fetch again per piece; cached accesses let the
+    // L2 merge them (FP64 p = 1, 2, 4: 12%, 16%, 8% faster per launch).
+    constexpr bool NT_SLAB = (NQ2 % 16 == 0) || NQ > 7;
+    if (active && use_q0) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k)
+        q0v[k] = NT_SLAB ? __builtin_nontemporal_load(q0_ + base + (int64_t)k * NQ2)
+                    : q0_[base + (int64_t)k * NQ2];
+    }
+    // D and the 1D Lift go to LDS last: the wait for them comes after every
+    // other load is in flight
+    for (int t = tid; t < NQ2; t += C::THREADS) sD[t] = D_[t];
+    for (int t = tid; t < 2 * NQ; t += C::THREADS) sL[t] = ell_[t];
+    __builtin_amdgcn_s_waitcnt(0);  // the async copies have landed in LDS
+    __syncthreads();
 
-  // ---- 4. contractions, Lift, combine, epilogue -----------------------------
-  if (active) {
-    double Di[NQ], Dj[NQ];
+    // ---- 2. upwind flux of face node (i,j) on each face ----------------------
+    double gf[6];
+    if (active) {
 #pragma unroll
-    for (int l = 0; l < NQ; ++l) {
-      Di[l] = sD[i + l * NQ];
-      Dj[l] = sD[j + l * NQ];
+      for (int f = 0; f < 6; ++f) {
+        if (flux_in_) {
+          gf[f] = flux_in_[e * NFT + f * NQ2 + ij];
+        } else {
+          const double n1 = g[kGeoNormal + 3 * f], n2 = g[kGeoNormal + 3 * f + 1],
+                       n3 = g[kGeoNormal + 3 * f + 2];
+          const int m = lb + own_face_node(f, i, j, NQ);
+          const double qM = sq[m];
+          double VelM, VelP;
+          if (axal) {
+            double vm;
+            if constexpr (CVEL)
+              vm = cvel[kFaceAxis[f]];
+            else if constexpr (VREG)
+              vm = kFaceAxis[f] == 0 ? su[m] : f == 0 ? vM[0] : f == 2 ? vM[1]
+                   : f == 4 ? wc[0] : wc[NQ - 1];
+            else
+              vm = (kFaceAxis[f] == 0 ? su : kFaceAxis[f] == 1 ? sv : sw)[m];
+            const double na = g[kGeoNormal + 3 * f + kFaceAxis[f]];
+            VelM = vm * na;
+            VelP = uP[f] * na;
+          } else {
+            if constexpr (VREG) {
+              const int64_t gm = e * NP + own_face_node(f, i, j, NQ);
+              VelM = su[m] * n1 + v_[gm] * n2 + w_[gm] * n3;
+            } else {
+              VelM = su[m] * n1 + sv[m] * n2 + sw[m] * n3;
+            }
+            VelP = uP[f] * n1 + vP[f] * n2 + wP[f] * n3;
+          }
+          const double alpha = 0.5 * fabs(VelP + VelM);
+          gf[f] = 0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qM * VelM - alpha * (qP[f] - qM));
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < 4; ++f) sG[(el * 4 + f) * NQ2 + ij] = gf[f];
     }
-    const double Ex = g[kGeoEscale], Ey = g[kGeoEscale + 1], Ez = g[kGeoEscale + 2];
-    const double lmi = sL[i], lpi = sL[NQ + i], lmj = sL[j], lpj = sL[NQ + j];
-    const double *G = sG + el * 4 * NQ2;
+    __syncthreads();
+
+    // ---- 3. volume fluxes: Fx, Fy in LDS, Fz and q in registers ---------------
+    double qc[NQ], fz[NQ];
+    if (active) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      double dx = 0.0, dy = 0.0, dz = 0.0;
+      for (int k = 0; k < NQ; ++k) {
+        const int n = lb + ij + k * NQ2;
+        const double qq = sq[n], uu = CVEL ? cvel[0] : su[n];
+        const double vv = CVEL ? cvel[1] : VREG ? vc[k] : sv[n];
+        const double ww = CVEL ? cvel[2] : VREG ? wc[k] : sw[n];
+        qc[k] = qq;
+        fz[k] = qq * ww;
+        if (C::FX_T) su[lb + j + i * NQ + k * NQ2] = qq * uu;
+        else su[n] = qq * uu;
+        (VREG ? sq : sv)[n] = qq * vv;  // Fy (in place of q with VREG)
+      }
+    }
+    __syncthreads();
+
+    // ---- 4. contractions, Lift, combine, epilogue -----------------------------
+    if (active) {
+      double Di[NQ], Dj[NQ];
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        dx += Di[l] * (C::FX_T ? su[lb + j + l * NQ + k * NQ2] : su[lb + l + j * NQ + k * NQ2]);
-        dy += Dj[l] * (VREG ? sq : sv)[lb + i + l * NQ + k * NQ2];
-        dz += sD[k + l * NQ] * fz[l];  // wave-uniform LDS broadcast
+        Di[l] = sD[i + l * NQ];
+        Dj[l] = sD[j + l * NQ];
       }
-      const double lift = lmj * G[0 * NQ2 + i + k * NQ] + lpi * G[1 * NQ2 + j + k * NQ] +
-                          lpj * G[2 * NQ2 + i + k * NQ] + lmi * G[3 * NQ2 + j + k * NQ] +
-                          sL[k] * gf[4] + sL[NQ + k] * gf[5];
-      const double tend = -(((Ex * dx + Ey * dy) + Ez * dz) + lift);
-      const int64_t idx = base + (int64_t)k * NQ2;
-      double res = tend;
-      if (rk) {
-        const double upd = cb_ * (qc[k] + (dt_geo_ ? g[kGeoDt] : dt_) * tend);
-        res = use_q0 ? ca_ * q0v[k] + upd : upd;
+      const double Ex = g[kGeoEscale], Ey = g[kGeoEscale + 1], Ez = g[kGeoEscale + 2];
+      const double lmi = sL[i], lpi = sL[NQ + i], lmj = sL[j], lpj = sL[NQ + j];
+      const double *G = sG + el * 4 * NQ2;
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) {
+          dx += Di[l] * (C::FX_T ? su[lb + j + l * NQ + k * NQ2] : su[lb + l + j * NQ + k * NQ2]);
+          dy += Dj[l] * (VREG ? sq : sv)[lb + i + l * NQ + k * NQ2];
+          dz += sD[k + l * NQ] * fz[l];  // wave-uniform LDS broadcast
+        }
+        const double lift = lmj * G[0 * NQ2 + i + k * NQ] + lpi * G[1 * NQ2 + j + k * NQ] +
+                            lpj * G[2 * NQ2 + i + k * NQ] + lmi * G[3 * NQ2 + j + k * NQ] +
+                            sL[k] * gf[4] + sL[NQ + k] * gf[5];
+        const double tend = -(((Ex * dx + Ey * dy) + Ez * dz) + lift);
+        const int64_t idx = base + (int64_t)k * NQ2;
+        double res = tend;
+        if (rk) {
+          const double upd = cb_ * (qc[k] + (dt_geo_ ? g[kGeoDt] : dt_) * tend);
+          res = use_q0 ? ca_ * q0v[k] + upd : upd;
+        }
+        if constexpr (NT_SLAB) __builtin_nontemporal_store(res, out_ + idx);
+        else out_[idx] = res;
       }
-      if constexpr (NT_SLAB) __builtin_nontemporal_store(res, out_ + idx);
-      else out_[idx] = res;
     }
-  }
   };
   if constexpr (GU) body(geo_);
   else body(g_lane);

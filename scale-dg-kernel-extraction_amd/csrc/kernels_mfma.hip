@@ -261,308 +261,322 @@ stage_mfma(const double *__restrict__ q_, const double *q0_,
   }
   static_assert(!GU || EPB > 1, "one element per workgroup reads its record uniformly anyway");
   auto body = [&](const double *__restrict__ g) {
-  const int64_t ebase = e * NP;
-  const int64_t base = ebase + ij;
-  const bool rk = mode_ == 1;
-  const bool use_q0 = rk && ca_ != 0.0;
-  const bool axal = C::AX_RUNTIME ? axis_aligned_ != 0 : AXAL;
-  const int R = el * NQ2;
-  const vel_t *const velp[3] = {u_, v_, w_};
-  vel_t cv[3] = {(vel_t)0, (vel_t)0, (vel_t)0};
-  if constexpr (CVEL) {
-    cv[0] = (vel_t)g[kGeoVel];
-    cv[1] = (vel_t)g[kGeoVel + 1];
-    cv[2] = (vel_t)g[kGeoVel + 2];
-  }
+    const int64_t ebase = e * NP;
+    const int64_t base = ebase + ij;
+    const bool rk = mode_ == 1;
+    const bool use_q0 = rk && ca_ != 0.0;
+    const bool axal = C::AX_RUNTIME ? axis_aligned_ != 0 : AXAL;
+    const int R = el * NQ2;
+    const vel_t *const velp[3] = {u_, v_, w_};
+    vel_t cv[3] = {(vel_t)0, (vel_t)0, (vel_t)0};
+    if constexpr (CVEL) {
+      cv[0] = (vel_t)g[kGeoVel];
+      cv[1] = (vel_t)g[kGeoVel + 1];
+      cv[2] = (vel_t)g[kGeoVel + 2];
+    }
 
-  // ---- 1. loads straight into registers ------------------------------------
-  // Neighbour-table entries: with several elements per workgroup they are
-  // per-lane loads, issued first so that waiting for them before the trace
-  // gathers does not also wait for the column loads behind them; with one
-  // they are scalar loads read where used (fewer live registers at high
-  // order).
-  int32_t nb6[6] = {};
-  const bool resolved = vmapP_ != nullptr;
-  if constexpr (EPB > 1) {
-    if (active) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
-  }
-  auto nbr_at = [&](int f) -> int64_t {
-    if constexpr (EPB > 1) return nbr_node<NQ>(resolved, nb6[f], f, i, j);
-    return resolved ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
-                    : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
-  };
-  double qc[NQ];
-  vel_t uc[SEQ ? 1 : NQ], vc[SEQ ? 1 : NQ], wc[NQ];  // SEQ: w only at k = 0, NQ-1
-  double qM[4], qP[6];
-  vel_t vM[4], vP[6];  // velocity component along the face axis
-  if (active) {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      qc[k] = q_[base + k * NQ2];
-      if constexpr (!SEQ) {
-        uc[k] = CVEL ? cv[0] : u_[base + k * NQ2];
-        vc[k] = CVEL ? cv[1] : v_[base + k * NQ2];
-        wc[k] = CVEL ? cv[2] : w_[base + k * NQ2];
-      }
+    // ---- 1. loads straight into registers ------------------------------------
+    // Neighbour-table entries: with several elements per workgroup they are
+    // per-lane loads, issued first so that waiting for them before the trace
+    // gathers does not also wait for the column loads behind them; with one
+    // they are scalar loads read where used (fewer live registers at high
+    // order).
+    int32_t nb6[6] = {};
+    const bool resolved = vmapP_ != nullptr;
+    if constexpr (EPB > 1) {
+      if (active) nbr_entries<NQ>(vmapP_, nbr_, e, ij, nb6);
     }
-    if constexpr (SEQ) {
-      wc[0] = CVEL ? cv[2] : w_[base];
-      wc[NQ - 1] = CVEL ? cv[2] : w_[base + (NQ - 1) * NQ2];
-    }
-    if constexpr (!C::XCH && !C::XD) {
-#pragma unroll
-      for (int f = 0; f < 4; ++f) {  // own traces of the x and y faces
-        const int64_t m = ebase + own_face_node(f, i, j, NQ);
-        qM[f] = q_[m];
-        vM[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? u_ : v_)[m];
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-      const int64_t pidx = nbr_at(f);
-      qP[f] = q_[pidx];
-      vP[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_)[pidx];
-    }
-  }
-
-  // ---- 1b. own x/y face traces from the lanes that hold them (XCH) --------
-  // Face node (a,b) of face f lies in the column of lane (a,0), (NQ-1,a),
-  // (a,NQ-1) or (0,a) at k = b; every lane stores its q, u, v columns in LDS
-  // and reads its four traces back, with no global loads (they would be
-  // gathers of one cache line per two lanes on the x faces).
-  if constexpr (C::XCH) {
-    double *xq = (double *)smem_mp;
-    vel_t *xu = (vel_t *)(xq + C::LANES * NQ), *xv = xu + C::LANES * NQ;
+    auto nbr_at = [&](int f) -> int64_t {
+      if constexpr (EPB > 1) return nbr_node<NQ>(resolved, nb6[f], f, i, j);
+      return resolved ? (int64_t)vmapP_[e * NFT + f * NQ2 + ij]
+                      : (int64_t)nbr_[e * 6 + f] * NP + nbr_face_node(f, i, j, NQ);
+    };
+    double qc[NQ];
+    vel_t uc[SEQ ? 1 : NQ], vc[SEQ ? 1 : NQ], wc[NQ];  // SEQ: w only at k = 0, NQ-1
+    double qM[4], qP[6];
+    vel_t vM[4], vP[6];  // velocity component along the face axis
     if (active) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
-        xq[tid * NQ + k] = qc[k];
-        if constexpr (!CVEL) {
-          xu[tid * NQ + k] = uc[k];
-          xv[tid * NQ + k] = vc[k];
+        qc[k] = q_[base + k * NQ2];
+        if constexpr (!SEQ) {
+          uc[k] = CVEL ? cv[0] : u_[base + k * NQ2];
+          vc[k] = CVEL ? cv[1] : v_[base + k * NQ2];
+          wc[k] = CVEL ? cv[2] : w_[base + k * NQ2];
         }
       }
-    }
-    __syncthreads();
-    if (active) {
-      const int src[4] = {R + i, R + (NQ - 1) + i * NQ, R + i + (NQ - 1) * NQ, R + i * NQ};
-#pragma unroll
-      for (int f = 0; f < 4; ++f) {
-        qM[f] = xq[src[f] * NQ + j];
-        vM[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? xu : xv)[src[f] * NQ + j];
+      if constexpr (SEQ) {
+        wc[0] = CVEL ? cv[2] : w_[base];
+        wc[NQ - 1] = CVEL ? cv[2] : w_[base + (NQ - 1) * NQ2];
       }
-    }
-    __syncthreads();  // the A operands below reuse the area
-  }
-
-  double *xd_q = (double *)((char *)smem_mp + C::A_BYTES) + el * 4 * NQ2;
-  vel_t *xd_v = (vel_t *)((double *)((char *)smem_mp + C::A_BYTES) + EPB * 4 * NQ2) + el * 4 * NQ2;
-  if constexpr (C::XD) {
-    if (active) {
+      if constexpr (!C::XCH && !C::XD) {
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) {
-        if (j == 0) { xd_q[0 * NQ2 + i + k * NQ] = qc[k]; xd_v[0 * NQ2 + i + k * NQ] = vc[k]; }
-        if (i == NQ - 1) { xd_q[1 * NQ2 + j + k * NQ] = qc[k]; xd_v[1 * NQ2 + j + k * NQ] = uc[k]; }
-        if (j == NQ - 1) { xd_q[2 * NQ2 + i + k * NQ] = qc[k]; xd_v[2 * NQ2 + i + k * NQ] = vc[k]; }
-        if (i == 0) { xd_q[3 * NQ2 + j + k * NQ] = qc[k]; xd_v[3 * NQ2 + j + k * NQ] = uc[k]; }
+        for (int f = 0; f < 4; ++f) {  // own traces of the x and y faces
+          const int64_t m = ebase + own_face_node(f, i, j, NQ);
+          qM[f] = q_[m];
+          vM[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? u_ : v_)[m];
+        }
       }
-    }
-  }
-  // ---- 2a. FP64 upwind fluxes -------------------------------------------------
-  op_t fl[6];
-  auto face_flux = [&](int f) {
-    {
-      const double qm = f == 4 ? qc[0] : f == 5 ? qc[NQ - 1] : qM[f];
-      double VelM, VelP;
-      if (axal) {
-        const int ax = kFaceAxis[f];
-        const double na = g[kGeoNormal + 3 * f + ax];
-        const vel_t vm = f == 4 ? wc[0] : f == 5 ? wc[NQ - 1] : vM[f];
-        VelM = (double)vm * na;
-        VelP = (double)vP[f] * na;
-      } else {
-        // general normals: all three components on both sides
-        const int64_t m = ebase + own_face_node(f, i, j, NQ);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
         const int64_t pidx = nbr_at(f);
-        const double n1 = g[kGeoNormal + 3 * f], n2 = g[kGeoNormal + 3 * f + 1],
-                     n3 = g[kGeoNormal + 3 * f + 2];
-        if constexpr (CVEL) {
-          VelM = (double)cv[0] * n1 + (double)cv[1] * n2 + (double)cv[2] * n3;
-          VelP = VelM;
-          (void)m;
-          (void)pidx;
-        } else {
-          VelM = (double)u_[m] * n1 + (double)v_[m] * n2 + (double)w_[m] * n3;
-          VelP = (double)u_[pidx] * n1 + (double)v_[pidx] * n2 + (double)w_[pidx] * n3;
-        }
+        qP[f] = q_[pidx];
+        vP[f] = CVEL ? cv[kFaceAxis[f]]
+                     : (kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_)[pidx];
       }
-      const double alpha = 0.5 * fabs(VelP + VelM);
-      fl[f] = (op_t)(0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qm * VelM - alpha * (qP[f] - qm)));
     }
-  };
-  if (active) {
-#pragma unroll
-    for (int f = C::XD ? 4 : 0; f < 6; ++f) face_flux(f);
-  }
 
-  // B = [D^T ; l- ; l+ ; 0] rows of this lane (k permuted, see above)
-  constexpr int NT = C::NT;
-  const int col = lane & 15, kq = lane >> 4;
-  op_t bop[NT][KSTEPS];
-#pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    const int c = 16 * nt + col;
-#pragma unroll
-    for (int ks = 0; ks < KSTEPS; ++ks) {
-      const int l = kq * KSTEPS + ks;
-      bop[nt][ks] = c >= NQ ? (op_t)0 : l < NQ ? D_[c + l * NQ] : l == NQ ? ell_[c]
-                    : l == NQ + 1 ? ell_[NQ + c] : (op_t)0;
-    }
-  }
-  const int rd = col * KPAD + kq * KSTEPS;        // A read offset of this lane in a tile
-  const int wr = P::RSTRIDE_Q * kq * KPAD + col;  // result offset of value 0
-  // Node (i,j,k) sits in the x pencil (j,k) at position i, in the y pencil
-  // (i,k) at position j and in the z pencil (i,j) at position k; face node
-  // (i,j) of the faces normal to d is pencil ij of direction d.
-  const int pos_off[3] = {(R + j) * KPAD + i, (R + i) * KPAD + j, (R + ij) * KPAD};
-  const int pos_step[3] = {NQ * KPAD, NQ * KPAD, 1};
-  // non-temporal RK base and output where k-slabs are whole 128-byte lines;
-  // cached where one instruction covers pieces of lines (p <= 6), so the L2
-  // merges them (as stage_f64; mixed p = 2, 4: 6% faster per launch)
-  constexpr bool NT_SLAB = (NQ2 % 16 == 0) || NQ > 7;
-  const int face_m[3] = {3, 0, 4}, face_p[3] = {1, 2, 5};  // -d and +d faces
-
-  double q0v[NQ];
-  double tend[NQ];
-  if constexpr (!SEQ) {
-    // ---- 2b. volume fluxes of all three directions -> A arrays ------------
-    if (active) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double Ed = g[kGeoEscale + d];
-        op_t *a = As + d * AREA + pos_off[d];
+    // ---- 1b. own x/y face traces from the lanes that hold them (XCH) --------
+    // Face node (a,b) of face f lies in the column of lane (a,0), (NQ-1,a),
+    // (a,NQ-1) or (0,a) at k = b; every lane stores its q, u, v columns in LDS
+    // and reads its four traces back, with no global loads (they would be
+    // gathers of one cache line per two lanes on the x faces).
+    if constexpr (C::XCH) {
+      double *xq = (double *)smem_mp;
+      vel_t *xu = (vel_t *)(xq + C::LANES * NQ), *xv = xu + C::LANES * NQ;
+      if (active) {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-          const vel_t vel = d == 0 ? uc[k] : d == 1 ? vc[k] : wc[k];
-          a[k * pos_step[d]] = (op_t)(Ed * (qc[k] * (double)vel));
+          xq[tid * NQ + k] = qc[k];
+          if constexpr (!CVEL) {
+            xu[tid * NQ + k] = uc[k];
+            xv[tid * NQ + k] = vc[k];
+          }
         }
-        op_t *r = As + d * AREA + (R + ij) * KPAD;
-        if (!C::XD || d == 2) {
-          r[NQ] = fl[face_m[d]];
-          r[NQ + 1] = fl[face_p[d]];
-        }
+      }
+      __syncthreads();
+      if (active) {
+        const int src[4] = {R + i, R + (NQ - 1) + i * NQ, R + i + (NQ - 1) * NQ, R + i * NQ};
 #pragma unroll
-        for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
+        for (int f = 0; f < 4; ++f) {
+          qM[f] = xq[src[f] * NQ + j];
+          vM[f] = CVEL ? cv[kFaceAxis[f]] : (kFaceAxis[f] == 0 ? xu : xv)[src[f] * NQ + j];
+        }
+      }
+      __syncthreads();  // the A operands below reuse the area
+    }
+
+    double *xd_q = (double *)((char *)smem_mp + C::A_BYTES) + el * 4 * NQ2;
+    vel_t *xd_v =
+        (vel_t *)((double *)((char *)smem_mp + C::A_BYTES) + EPB * 4 * NQ2) + el * 4 * NQ2;
+    if constexpr (C::XD) {
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+          if (j == 0) {
+            xd_q[0 * NQ2 + i + k * NQ] = qc[k];
+            xd_v[0 * NQ2 + i + k * NQ] = vc[k];
+          }
+          if (i == NQ - 1) {
+            xd_q[1 * NQ2 + j + k * NQ] = qc[k];
+            xd_v[1 * NQ2 + j + k * NQ] = uc[k];
+          }
+          if (j == NQ - 1) {
+            xd_q[2 * NQ2 + i + k * NQ] = qc[k];
+            xd_v[2 * NQ2 + i + k * NQ] = vc[k];
+          }
+          if (i == 0) {
+            xd_q[3 * NQ2 + j + k * NQ] = qc[k];
+            xd_v[3 * NQ2 + j + k * NQ] = uc[k];
+          }
+        }
       }
     }
-    // RK base column, in flight during the GEMMs (the RK base and the output
-    // are touched once per launch: non-temporal, so L2 keeps the neighbour
-    // traces)
-    if (active && use_q0) {
+    // ---- 2a. FP64 upwind fluxes -------------------------------------------------
+    op_t fl[6];
+    auto face_flux = [&](int f) {
+      {
+        const double qm = f == 4 ? qc[0] : f == 5 ? qc[NQ - 1] : qM[f];
+        double VelM, VelP;
+        if (axal) {
+          const int ax = kFaceAxis[f];
+          const double na = g[kGeoNormal + 3 * f + ax];
+          const vel_t vm = f == 4 ? wc[0] : f == 5 ? wc[NQ - 1] : vM[f];
+          VelM = (double)vm * na;
+          VelP = (double)vP[f] * na;
+        } else {
+          // general normals: all three components on both sides
+          const int64_t m = ebase + own_face_node(f, i, j, NQ);
+          const int64_t pidx = nbr_at(f);
+          const double n1 = g[kGeoNormal + 3 * f], n2 = g[kGeoNormal + 3 * f + 1],
+                       n3 = g[kGeoNormal + 3 * f + 2];
+          if constexpr (CVEL) {
+            VelM = (double)cv[0] * n1 + (double)cv[1] * n2 + (double)cv[2] * n3;
+            VelP = VelM;
+            (void)m;
+            (void)pidx;
+          } else {
+            VelM = (double)u_[m] * n1 + (double)v_[m] * n2 + (double)w_[m] * n3;
+            VelP = (double)u_[pidx] * n1 + (double)v_[pidx] * n2 + (double)w_[pidx] * n3;
+          }
+        }
+        const double alpha = 0.5 * fabs(VelP + VelM);
+        fl[f] = (op_t)(0.5 * g[kGeoFscale + f] * (qP[f] * VelP - qm * VelM - alpha * (qP[f] - qm)));
+      }
+    };
+    if (active) {
+#pragma unroll
+      for (int f = C::XD ? 4 : 0; f < 6; ++f) face_flux(f);
+    }
+
+    // B = [D^T ; l- ; l+ ; 0] rows of this lane (k permuted, see above)
+    constexpr int NT = C::NT;
+    const int col = lane & 15, kq = lane >> 4;
+    op_t bop[NT][KSTEPS];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int c = 16 * nt + col;
+#pragma unroll
+      for (int ks = 0; ks < KSTEPS; ++ks) {
+        const int l = kq * KSTEPS + ks;
+        bop[nt][ks] = c >= NQ ? (op_t)0 : l < NQ ? D_[c + l * NQ] : l == NQ ? ell_[c]
+                      : l == NQ + 1 ? ell_[NQ + c] : (op_t)0;
+      }
+    }
+    const int rd = col * KPAD + kq * KSTEPS;        // A read offset of this lane in a tile
+    const int wr = P::RSTRIDE_Q * kq * KPAD + col;  // result offset of value 0
+    // Node (i,j,k) sits in the x pencil (j,k) at position i, in the y pencil
+    // (i,k) at position j and in the z pencil (i,j) at position k; face node
+    // (i,j) of the faces normal to d is pencil ij of direction d.
+    const int pos_off[3] = {(R + j) * KPAD + i, (R + i) * KPAD + j, (R + ij) * KPAD};
+    const int pos_step[3] = {NQ * KPAD, NQ * KPAD, 1};
+    // non-temporal RK base and output where k-slabs are whole 128-byte lines;
+    // cached where one instruction covers pieces of lines (p <= 6), so the L2
+    // merges them (as stage_f64; mixed p = 2, 4: 6% faster per launch)
+    constexpr bool NT_SLAB = (NQ2 % 16 == 0) || NQ > 7;
+    const int face_m[3] = {3, 0, 4}, face_p[3] = {1, 2, 5};  // -d and +d faces
+
+    double q0v[NQ];
+    double tend[NQ];
+    if constexpr (!SEQ) {
+      // ---- 2b. volume fluxes of all three directions -> A arrays ------------
+      if (active) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double Ed = g[kGeoEscale + d];
+          op_t *a = As + d * AREA + pos_off[d];
+#pragma unroll
+          for (int k = 0; k < NQ; ++k) {
+            const vel_t vel = d == 0 ? uc[k] : d == 1 ? vc[k] : wc[k];
+            a[k * pos_step[d]] = (op_t)(Ed * (qc[k] * (double)vel));
+          }
+          op_t *r = As + d * AREA + (R + ij) * KPAD;
+          if (!C::XD || d == 2) {
+            r[NQ] = fl[face_m[d]];
+            r[NQ + 1] = fl[face_p[d]];
+          }
+#pragma unroll
+          for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
+        }
+      }
+      // RK base column, in flight during the GEMMs (the RK base and the output
+      // are touched once per launch: non-temporal, so L2 keeps the neighbour
+      // traces)
+      if (active && use_q0) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+          q0v[k] = NT_SLAB ? __builtin_nontemporal_load(q0_ + base + k * NQ2) : q0_[base + k * NQ2];
+      }
+      __syncthreads();
+      if constexpr (C::XD) {
+        if (active) {
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            qM[f] = xd_q[f * NQ2 + ij];
+            vM[f] = CVEL ? cv[kFaceAxis[f]] : xd_v[f * NQ2 + ij];
+            face_flux(f);
+          }
+#pragma unroll
+          for (int d = 0; d < 2; ++d) {
+            op_t *r = As + d * AREA + (R + ij) * KPAD;
+            r[NQ] = fl[face_m[d]];
+            r[NQ + 1] = fl[face_p[d]];
+          }
+        }
+        __syncthreads();
+      }
+
+      // ---- 3. MFMA GEMMs over (direction, 16-pencil tile) jobs ---------------
+      // Rows of A beyond the block's active pencils hold stale LDS; each output
+      // row depends only on its own A row, and those rows are never read back.
+#pragma unroll
+      for (int jj = 0; jj < C::JPW; ++jj) {
+        const int job = wave + jj * C::WAVES;
+        if (job >= C::JOBS) break;  // wave-uniform
+        const int d = job / C::MTILES, t = job - d * C::MTILES;
+        mfma_tile<P, KSTEPS, KPAD, NQ, NT>(As + d * AREA + t * 16 * KPAD, bop, rd, wr, col);
+      }
+      __syncthreads();
+
+      // ---- 4a. FP64 sum of the three directions ------------------------------
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+          tend[k] = ((double)As[pos_off[0] + k * pos_step[0]] +
+                     (double)As[AREA + pos_off[1] + k * pos_step[1]]) +
+                    (double)As[2 * AREA + pos_off[2] + k * pos_step[2]];
+      }
+    } else {
+      // ---- 2b-4a, one direction at a time through one LDS area ---------------
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (active) {
+          const double Ed = g[kGeoEscale + d];
+          const vel_t *vel = velp[d];
+          op_t *a = As + pos_off[d];
+#pragma unroll
+          for (int k = 0; k < NQ; ++k)
+            a[k * pos_step[d]] =
+                (op_t)(Ed * (qc[k] * (double)(CVEL ? cv[d] : vel[base + k * NQ2])));
+          op_t *r = As + (R + ij) * KPAD;
+          r[NQ] = fl[face_m[d]];
+          r[NQ + 1] = fl[face_p[d]];
+#pragma unroll
+          for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < C::JPW; ++jj) {
+          const int t = wave + jj * C::WAVES;
+          if (t >= C::MTILES) break;  // wave-uniform
+          mfma_tile<P, KSTEPS, KPAD, NQ, NT>(As + t * 16 * KPAD, bop, rd, wr, col);
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+          for (int k = 0; k < NQ; ++k) {
+            const double x = (double)As[pos_off[d] + k * pos_step[d]];
+            tend[k] = d == 0 ? x : tend[k] + x;
+          }
+        }
+        if (d < 2) __syncthreads();  // the next direction overwrites the area
+      }
+    }
+
+    // ---- 4b. RK epilogue (FP64) -------------------------------------------------
+    // SEQ reads the RK base only here: held through the last direction's GEMM it
+    // cost p = 16, 17 a wave per SIMD (FP64 Nq = 17: 154 -> 124 VGPRs; p = 16
+    // 399 -> 313 us per launch, p = 17 344 -> 276, mixed p = 19 271 -> 208)
+    if (SEQ && active && use_q0) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k)
         q0v[k] = NT_SLAB ? __builtin_nontemporal_load(q0_ + base + k * NQ2) : q0_[base + k * NQ2];
     }
-    __syncthreads();
-    if constexpr (C::XD) {
-      if (active) {
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          qM[f] = xd_q[f * NQ2 + ij];
-          vM[f] = CVEL ? cv[kFaceAxis[f]] : xd_v[f * NQ2 + ij];
-          face_flux(f);
-        }
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          op_t *r = As + d * AREA + (R + ij) * KPAD;
-          r[NQ] = fl[face_m[d]];
-          r[NQ + 1] = fl[face_p[d]];
-        }
-      }
-      __syncthreads();
-    }
-
-    // ---- 3. MFMA GEMMs over (direction, 16-pencil tile) jobs ---------------
-    // Rows of A beyond the block's active pencils hold stale LDS; each output
-    // row depends only on its own A row, and those rows are never read back.
-#pragma unroll
-    for (int jj = 0; jj < C::JPW; ++jj) {
-      const int job = wave + jj * C::WAVES;
-      if (job >= C::JOBS) break;  // wave-uniform
-      const int d = job / C::MTILES, t = job - d * C::MTILES;
-      mfma_tile<P, KSTEPS, KPAD, NQ, NT>(As + d * AREA + t * 16 * KPAD, bop, rd, wr, col);
-    }
-    __syncthreads();
-
-    // ---- 4a. FP64 sum of the three directions ------------------------------
     if (active) {
+      const double dtv = dt_geo_ ? g[kGeoDt] : dt_;
 #pragma unroll
-      for (int k = 0; k < NQ; ++k)
-        tend[k] = ((double)As[pos_off[0] + k * pos_step[0]] +
-                   (double)As[AREA + pos_off[1] + k * pos_step[1]]) +
-                  (double)As[2 * AREA + pos_off[2] + k * pos_step[2]];
-    }
-  } else {
-    // ---- 2b-4a, one direction at a time through one LDS area ---------------
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (active) {
-        const double Ed = g[kGeoEscale + d];
-        const vel_t *vel = velp[d];
-        op_t *a = As + pos_off[d];
-#pragma unroll
-        for (int k = 0; k < NQ; ++k)
-          a[k * pos_step[d]] =
-              (op_t)(Ed * (qc[k] * (double)(CVEL ? cv[d] : vel[base + k * NQ2])));
-        op_t *r = As + (R + ij) * KPAD;
-        r[NQ] = fl[face_m[d]];
-        r[NQ + 1] = fl[face_p[d]];
-#pragma unroll
-        for (int l = NQ + 2; l < KPAD; ++l) r[l] = (op_t)0;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int jj = 0; jj < C::JPW; ++jj) {
-        const int t = wave + jj * C::WAVES;
-        if (t >= C::MTILES) break;  // wave-uniform
-        mfma_tile<P, KSTEPS, KPAD, NQ, NT>(As + t * 16 * KPAD, bop, rd, wr, col);
-      }
-      __syncthreads();
-      if (active) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) {
-          const double x = (double)As[pos_off[d] + k * pos_step[d]];
-          tend[k] = d == 0 ? x : tend[k] + x;
+      for (int k = 0; k < NQ; ++k) {
+        const double td = -tend[k];
+        const int64_t idx = base + (int64_t)k * NQ2;
+        double res = td;
+        if (rk) {
+          const double upd = cb_ * (qc[k] + dtv * td);
+          res = use_q0 ? ca_ * q0v[k] + upd : upd;
         }
+        if constexpr (NT_SLAB) __builtin_nontemporal_store(res, out_ + idx);
+        else out_[idx] = res;
       }
-      if (d < 2) __syncthreads();  // the next direction overwrites the area
     }
-  }
-
-  // ---- 4b. RK epilogue (FP64) -------------------------------------------------
-  // SEQ reads the RK base only here: held through the last direction's GEMM it
-  // cost p = 16, 17 a wave per SIMD (FP64 Nq = 17: 154 -> 124 VGPRs; p = 16
-  // 399 -> 313 us per launch, p = 17 344 -> 276, mixed p = 19 271 -> 208)
-  if (SEQ && active && use_q0) {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k)
-      q0v[k] = NT_SLAB ? __builtin_nontemporal_load(q0_ + base + k * NQ2) : q0_[base + k * NQ2];
-  }
-  if (active) {
-    const double dtv = dt_geo_ ? g[kGeoDt] : dt_;
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const double td = -tend[k];
-      const int64_t idx = base + (int64_t)k * NQ2;
-      double res = td;
-      if (rk) {
-        const double upd = cb_ * (qc[k] + dtv * td);
-        res = use_q0 ? ca_ * q0v[k] + upd : upd;
-      }
-      if constexpr (NT_SLAB) __builtin_nontemporal_store(res, out_ + idx);
-      else out_[idx] = res;
-    }
-  }
   };
   if constexpr (GU) body(geo_);
   else body(g_lane);
