@@ -273,7 +273,10 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  * mesh (writing only the solver's scratch rotation buffer) and keeps the
  * fastest for that solver alone, since the best shape depends on the mesh
  * (the default stays unless another shape is at least 2% faster);
- * us_per_launch[c] receives candidate c's time.  dgk_batch_autotune does this
+ * us_per_launch[c] receives candidate c's time (0 for a candidate that would
+ * launch another shape on this solver: the kernel over runs of elements, on
+ * launches it cannot take -- constant velocities, a resolved face-node table,
+ * an explicit order).  dgk_batch_autotune does this
  * for every order group of a batch and reports per group: the order, the
  * chosen candidate, its time and the default's time (the batch's captured
  * step is re-captured at the next dgk_batch_step).  No counterpart

@@ -170,21 +170,29 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
 // One stage launch on `stream` (the solver's own stream unless a batch
 // supplies one); dt_geo takes the time step from each element's geometry
 // record instead of `dt`.
+StageArgsMP stage_args_mp(dgk_solver *s, const double *qin, const double *q0,
+                          double *out, int mode, double ca, double cb, double dt) {
+  StageArgsMP a;
+  a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
+  a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
+  a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = 0;
+  a.zplane = s->zplane; a.Ne = s->Ne;
+  a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
+  a.dt_geo = 0;
+  a.tile = s->tile;
+  a.geo_per_run = s->geo_per_run;
+  a.cvel = s->cvel;
+  return a;
+}
+
 int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out,
                  int mode, double ca, double cb, double dt, hipStream_t stream, int dt_geo) {
   if (!stream) stream = s->stream;
   const int reverse = (s->alternate && (s->launches++ & 1)) ? 1 : 0;
   if (s->evaltype == DGK_EVAL_HIP_MP) {
-    StageArgsMP a;
-    a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
-    a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
-    a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = reverse;
-    a.zplane = s->zplane; a.Ne = s->Ne;
-    a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
+    StageArgsMP a = stage_args_mp(s, qin, q0, out, mode, ca, cb, dt);
+    a.reverse = reverse;
     a.dt_geo = dt_geo;
-    a.tile = s->tile;
-    a.geo_per_run = s->geo_per_run;
-    a.cvel = s->cvel;
     return launch_stage_mp(s->Nq, a, stream);
   }
   StageArgs a = stage_args(s, qin, q0, out, mode, ca, cb, dt);
@@ -1155,13 +1163,28 @@ int time_stage(dgk_solver *s, int reps, double *us) {
 
 constexpr double kTuneMargin = 0.02;
 
+// Whether candidate c launches its own kernel for s's stage launches.
+bool candidate_runs(dgk_solver *s, int c) {
+  double *A = s->q[s->cur], *C = s->q[(s->cur + 2) % 3];
+  if (s->evaltype == DGK_EVAL_HIP_MP) {
+    StageArgsMP a = stage_args_mp(s, A, A, C, 1, kRkA[1], kRkB[1], 0.0);
+    return stage_mp_candidate_runs(s->Nq, c, a);
+  }
+  return stage_f64_candidate_runs(s->Nq, c, stage_args(s, A, A, C, 1, kRkA[1], kRkB[1], 0.0));
+}
+
 int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
   int epb[kMaxTileShapes];
   const int nc = tile_shapes(s->evaltype, s->Nq, epb);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "no tile shapes for this form and order");
   double t[kMaxTileShapes] = {};
-  int arg = 0;
+  int arg = -1;
   for (int c = 0; c < nc; ++c) {
+    // a candidate that would launch another shape here is not timed (0)
+    if (!candidate_runs(s, c)) {
+      if (us && c < max) us[c] = 0.0;
+      continue;
+    }
     s->tile = c;
     const int rc = time_stage(s, reps, &t[c]);
     if (rc != DGK_OK) {
@@ -1169,11 +1192,12 @@ int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
       return rc;
     }
     if (us && c < max) us[c] = t[c];
-    if (t[c] < t[arg]) arg = c;
+    if (arg < 0 || t[c] < t[arg]) arg = c;
   }
+  if (arg < 0) return set_error(DGK_ERR_UNSUPPORTED, "no tile shape applies");
   // keep the default unless another shape is at least kTuneMargin faster
   const int def = tile_selected(s->evaltype, s->Nq);
-  if (def >= 0 && def < nc && t[arg] > (1.0 - kTuneMargin) * t[def]) arg = def;
+  if (def >= 0 && def < nc && t[def] > 0.0 && t[arg] > (1.0 - kTuneMargin) * t[def]) arg = def;
   s->tile = arg;
   if (best) *best = arg;
   if (n) *n = nc;
@@ -1242,7 +1266,7 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
       if (best) best[g] = arg;
       if (us_best) us_best[g] = t[arg];
       const int def = tile_selected(s->evaltype, s->Nq);
-      if (us_default) us_default[g] = t[def >= 0 && def < nc ? def : 0];
+      if (us_default) us_default[g] = t[def >= 0 && def < nc ? def : 0];  // 0: not applicable
     }
   }
   if (n) *n = (int)b->groups.size();

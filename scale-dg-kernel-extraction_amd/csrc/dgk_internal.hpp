@@ -261,6 +261,11 @@ struct StageArgsMP {
 
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s);
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
+// Whether tile candidate `cand` launches its own kernel for these arguments
+// (false: the run-kernel candidate on launches it cannot take, which run
+// another shape instead).
+bool stage_f64_candidate_runs(int Nq, int cand, const StageArgs &a);
+bool stage_mp_candidate_runs(int Nq, int cand, const StageArgsMP &a);
 // Tile shapes (elements per block) of the mixed-precision stage kernel of
 // order Nq (the MFMA shapes, then at p <= 4 the run kernel): fills epb and
 // returns their number (n >= 5).  stage_mp_select picks the

@@ -752,6 +752,13 @@ constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 
 std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1,
                                             3, 3, 0, 0, 0, 0};
 
+bool stage_f64_candidate_runs(int Nq, int cand, const StageArgs &a) {
+  if (Nq < kMinNq || Nq > kMaxNqValu) return true;
+  int mfma[kMaxTileShapes];
+  const int nm = tile_f64_shapes(Nq, mfma, kMaxTileShapes);
+  return !(cand == kValuShapes + nm && run_elements(Nq) > 0 && !run_applicable(Nq, a));
+}
+
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
   int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0

@@ -1050,6 +1050,12 @@ int prepare_stage_mp(int Nq) {
 #undef DGK_CALL
 }
 
+bool stage_mp_candidate_runs(int Nq, int cand, const StageArgsMP &a) {
+  int e[kMaxTileShapes];
+  return !(run_mp_elements(Nq) > 0 && cand == stage_mp_shapes(Nq, e, kMaxTileShapes) - 1 &&
+           !run_mp_applicable(Nq, a));
+}
+
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
   int cand = a.tile;

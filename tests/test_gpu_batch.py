@@ -117,3 +117,27 @@ def test_constant_and_field_velocities(form):
                 s.step(it["dt"], 4)
                 alone = s.get_field(dgk.FIELD_Q)
             assert np.array_equal(got[k], alone), (k, u0[0] == u0[-1])
+
+
+@pytest.mark.parametrize("form", ["HIP", "HIP_MP"])
+def test_autotune_skips_inapplicable_candidates(form):
+    # batched boxes with constant velocities cannot take the kernel over runs
+    # of elements (the last candidate at low order): the autotuner does not
+    # time it (0) on the batch, does on a plain solver, and never reports it
+    # for the batch
+    p = 3
+    last = len(dgk.tile_shapes(form, p)) - 1
+    ens = inputs.ensemble(n=4, seed=12, ne_choices=(4,), p_choices=(p,), nstep=2)
+    with dgk.Batch(form, ens) as b:
+        for k, it in enumerate(ens):
+            b.init_gaussian(k, it["centre"], it["sigma"], it["vel"])
+        b.step(1)
+        rec = b.autotune(reps=2)
+        assert [x["p"] for x in rec] == [p]
+        assert rec[0]["candidate"] != last and rec[0]["us_per_launch"] > 0
+        assert rec[0]["us_default"] > 0
+    wl = inputs.c1_sine_noise()
+    with dgk.Solver.structured(form, wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        r = s.autotune(reps=2)
+        assert len(r["us_per_launch"]) == last + 1 and all(t > 0 for t in r["us_per_launch"])
