@@ -386,9 +386,10 @@ int dgk_serve_destroy(dgk_server *sv);
  * No counterpart in the reference.  The HBM rate, in GB/s of bytes read plus
  * written, of a kernel that reads five arrays of n doubles and writes one --
  * the stage kernels' mix of about five bytes read per byte written -- with
- * 16-byte loads into LDS (gbs_async, as the stage kernels stage their
- * fields) and into registers (gbs_regs), non-temporal stores, averaged over
- * reps launches (after one warm-up) timed with HIP events.  n is a multiple of 512; the call
+ * 16-byte loads into LDS (gbs_async, one per lane and array, as the stage
+ * kernels stage their fields) and into registers (gbs_regs, four per lane and
+ * array), non-temporal stores, averaged over reps launches (after one
+ * warm-up) timed with HIP events.  n is a multiple of 2048; the call
  * allocates 48 n bytes of device memory and frees them before returning. */
 int dgk_calibrate_stream(int64_t n, int reps, double *gbs_async, double *gbs_regs);
 

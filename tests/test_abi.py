@@ -214,3 +214,13 @@ def test_shapes_above_the_operator_files():
         "stage_f64<4, true, true, false, false>")
     assert dgk.kernel_name("HIP_MP", 3, 1) == "stage_mfma<PrecMixed, 4, 8, false, true, false, true>"
     assert dgk.element_sizes(19) == dict(Nq=20, Np=8000, Nfp=400, NfpTot=2400)
+
+
+def test_calibrate_stream_arguments():
+    a, r = C.c_double(0.0), C.c_double(0.0)
+    # n must be a positive multiple of 2048 (four 16-byte granules per lane of
+    # a 256-lane workgroup in the register form); checked before any device use
+    for n in (0, 1000, 2048 + 512):
+        assert dgk.lib().dgk_calibrate_stream(n, 1, C.byref(a), C.byref(r)) == dgk.ERR_ARG
+    assert dgk.lib().dgk_calibrate_stream(2048, 0, C.byref(a), C.byref(r)) == dgk.ERR_ARG
+    assert dgk.lib().dgk_calibrate_stream(2048, 1, None, C.byref(r)) == dgk.ERR_ARG
