@@ -341,3 +341,30 @@ def test_general_face_normals_every_tile_shape(p):
             assert rel_l2 <= 1e-5, cand
     finally:
         dgk.tile_select("HIP_MP", p, -1)
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+def test_odd_node_count_staging_tail(p):
+    """Odd nodes per element (p = 2, 4, 6): the staged fields are copied in
+    aligned 16-byte granules, which for an even-numbered element reach 8 bytes
+    past it.  A mesh with an odd element count ends on such an element, so the
+    last granule of every staged field reads the allocation's tail pad; both
+    VALU forms, walking forward and backward over 4 steps, hold the oracle's
+    loop bar and give the same bits."""
+    ne = (3, 3, 3)
+    m = oracle.Mesh(*ne, p)
+    q, u, v, w = inputs.random_fields(*ne, p, seed=1300 + p, vel_scale=0.8)
+    dt = 0.2 * inputs.cfl_dt(inputs.lgl_nodes(p), ne, (0.8, 0.8, 0.8))
+    ref, _, _ = m.run(q, u, v, w, dt, 4)
+    got = []
+    try:
+        for cand in (0, 1):
+            dgk.tile_select("HIP", p, cand)
+            with dgk.Solver.structured("HIP", *ne, p) as s:
+                s.set_fields(q, u, v, w)
+                s.step(dt, 4)
+                got.append(s.get_field(dgk.FIELD_Q))
+            assert maxrel(got[-1], ref) <= LOOP_TOL, cand
+    finally:
+        dgk.tile_select("HIP", p, -1)
+    assert np.array_equal(got[0], got[1])
