@@ -136,9 +136,9 @@ def copy_rate(nbytes=1 << 30, reps=10):
 def stream_rate(dgk, n=1 << 24, reps=20):
     """dgk_calibrate_stream: a kernel reading five arrays of n doubles and
     writing one (the stage kernels' read:write mix), with 16-byte loads into
-    LDS (one per lane and array) and into registers (four per lane and
-    array, the fastest streaming form measured); GB/s of bytes read plus
-    written."""
+    LDS (one per lane and array) and 8-byte loads into registers (one per
+    lane and array, the fastest streaming form measured); GB/s of bytes read
+    plus written."""
     import ctypes as C
 
     a, r = C.c_double(0.0), C.c_double(0.0)

@@ -387,9 +387,10 @@ int dgk_serve_destroy(dgk_server *sv);
  * written, of a kernel that reads five arrays of n doubles and writes one --
  * the stage kernels' mix of about five bytes read per byte written -- with
  * 16-byte loads into LDS (gbs_async, one per lane and array, as the stage
- * kernels stage their fields) and into registers (gbs_regs, four per lane and
- * array), non-temporal stores, averaged over reps launches (after one
- * warm-up) timed with HIP events.  n is a multiple of 2048; the call
+ * kernels stage their fields) and 8-byte loads into registers (gbs_regs, one
+ * per lane and array, the fastest streaming form measured), non-temporal
+ * stores, averaged over reps launches (after one warm-up) timed with HIP
+ * events.  n is a multiple of 2048; the call
  * allocates 48 n bytes of device memory and frees them before returning. */
 int dgk_calibrate_stream(int64_t n, int reps, double *gbs_async, double *gbs_regs);
 

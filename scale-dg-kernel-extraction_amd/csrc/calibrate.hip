@@ -3,16 +3,20 @@
 //
 // A stage launch reads q, u, v, w (and q0 on two stages of three) and writes
 // one field: about five bytes read per byte written.  stream_probe<NR, ASYNC>
-// reads NR arrays and writes one (out = sum of the inputs), in the two load
-// styles the stage kernels use:
+// reads NR arrays and writes one (out = sum of the inputs), with
+// non-temporal stores as the stage kernels' outputs, in two load forms:
 //   ASYNC = true:  global_load_lds_dwordx4 into LDS (no VGPRs held), then LDS
 //                  reads, as the staged fields of stage_f64, 16 bytes per lane
 //                  per array;
-//   ASYNC = false: global_load_dwordx4 into registers, four 16-byte granules
-//                  per lane per array (of the one, two and four granules and
-//                  64..1024-lane workgroups measured, the fastest: 5.80 TB/s
-//                  against 5.43 with one granule).
-// Stores are non-temporal, as the stage kernels' outputs.
+//   ASYNC = false: one 8-byte load per lane and array into a register.  This
+//                  is the fastest streaming form measured on MI355X: 6.1-6.3
+//                  TB/s, against 5.4-5.9 for 16-byte loads (one to four per
+//                  lane), two to eight 8-byte loads per lane, 512- and
+//                  1024-lane workgroups, or cached stores.
+// Workgroups take consecutive spans in launch order, so the 8 XCDs (dealt
+// workgroups round-robin) work on neighbouring addresses at any time; giving
+// each XCD a contiguous eighth of the arrays instead costs 5-10% (5.5-5.7
+// TB/s in every load form).
 #include <algorithm>
 
 #include "dgk_internal.hpp"
@@ -21,18 +25,21 @@ namespace dgk {
 namespace {
 
 constexpr int kProbeThreads = 256;
-constexpr int kRegGranules = 4;                                  // per lane per array
-constexpr int kProbeDoubles = 2 * kProbeThreads * kRegGranules;  // per array per workgroup
+// doubles per array per workgroup: 16 bytes per lane (LDS form), 8 (register form)
+template <bool ASYNC>
+constexpr int kProbeSpan = ASYNC ? 2 * kProbeThreads : kProbeThreads;
+// n is a multiple of this (of both spans)
+constexpr int kProbeDoubles = 2048;
 
 template <int NR, bool ASYNC>
 __global__ void __launch_bounds__(kProbeThreads)
 stream_probe(const double *__restrict__ in, int64_t n, double *__restrict__ out) {
   // n is a multiple of kProbeDoubles (checked on the host); array r at in + r n
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  constexpr int PER = ASYNC ? 2 * kProbeThreads : kProbeDoubles;  // per array per workgroup
+  constexpr int PER = kProbeSpan<ASYNC>;
   const int64_t b0 = (int64_t)blockIdx.x * PER;
-  double2 acc = {0.0, 0.0};
   if constexpr (ASYNC) {
+    double2 acc = {0.0, 0.0};
     __shared__ double s[NR * PER];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -58,32 +65,19 @@ stream_probe(const double *__restrict__ in, int64_t n, double *__restrict__ out)
     __builtin_nontemporal_store(acc.x, o);
     __builtin_nontemporal_store(acc.y, o + 1);
   } else {
-    double2 a[kRegGranules];
+    double acc = 0.0;
 #pragma unroll
-    for (int g = 0; g < kRegGranules; ++g) a[g] = double2{0.0, 0.0};
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-#pragma unroll
-      for (int g = 0; g < kRegGranules; ++g) {
-        const double2 x =
-            *(const double2 *)(in + (int64_t)r * n + b0 + 2 * (g * kProbeThreads + tid));
-        a[g].x += x.x;
-        a[g].y += x.y;
-      }
-#pragma unroll
-    for (int g = 0; g < kRegGranules; ++g) {
-      double *o = out + b0 + 2 * (g * kProbeThreads + tid);
-      __builtin_nontemporal_store(a[g].x, o);
-      __builtin_nontemporal_store(a[g].y, o + 1);
-    }
-    (void)acc;
+    for (int r = 0; r < NR; ++r) acc += in[(int64_t)r * n + b0 + tid];
+    __builtin_nontemporal_store(acc, out + b0 + tid);
+    (void)wave;
+    (void)lane;
   }
 }
 
 template <int NR, bool ASYNC>
 int time_probe(const double *in, int64_t n, double *out, int reps, hipStream_t st,
                double *gbs) {
-  const int nblk = (int)(n / (ASYNC ? 2 * kProbeThreads : kProbeDoubles));
+  const int nblk = (int)(n / kProbeSpan<ASYNC>);
   hipEvent_t e0, e1;
   DGK_HIP(hipEventCreate(&e0));
   DGK_HIP(hipEventCreate(&e1));
@@ -116,7 +110,8 @@ extern "C" int dgk_calibrate_stream(int64_t n, int reps, double *gbs_async,
   if (n < kProbeDoubles || n % kProbeDoubles || reps < 1 || !gbs_async || !gbs_regs)
     return set_error(DGK_ERR_ARG, "n must be a positive multiple of " +
                                       std::to_string(kProbeDoubles) + ", reps >= 1");
-  if (n / kProbeDoubles > (int64_t)INT32_MAX) return set_error(DGK_ERR_ARG, "n too large");
+  if (n / kProbeSpan<false> > (int64_t)INT32_MAX)
+    return set_error(DGK_ERR_ARG, "n too large");
   constexpr int NR = 5;
   double *in = nullptr, *out = nullptr;
   hipStream_t st = nullptr;

@@ -218,8 +218,8 @@ def test_shapes_above_the_operator_files():
 
 def test_calibrate_stream_arguments():
     a, r = C.c_double(0.0), C.c_double(0.0)
-    # n must be a positive multiple of 2048 (four 16-byte granules per lane of
-    # a 256-lane workgroup in the register form); checked before any device use
+    # n must be a positive multiple of 2048 (a multiple of both forms' spans per
+    # 256-lane workgroup); checked before any device use
     for n in (0, 1000, 2048 + 512):
         assert dgk.lib().dgk_calibrate_stream(n, 1, C.byref(a), C.byref(r)) == dgk.ERR_ARG
     assert dgk.lib().dgk_calibrate_stream(2048, 0, C.byref(a), C.byref(r)) == dgk.ERR_ARG
