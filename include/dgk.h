@@ -255,16 +255,19 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  *           elements, then one wavefront processed direction by direction
  *           (a third of the LDS), for p <= 7; for p >= 8 one element, two
  *           elements, and one element direction by direction; above p = 15
- *           only the last.  Defaults: two wavefronts (candidate 1) at
- *           p = 1, 4, four wavefronts (candidate 2) at p = 5, the
- *           direction-by-direction shape at p = 14, candidate 0 otherwise.
+ *           one element, all at once and direction by direction.  Shapes
+ *           are limited to 160 KiB of LDS.  Defaults: two wavefronts
+ *           (candidate 1) at p = 1, 4, four wavefronts (candidate 2) at
+ *           p = 5, the direction-by-direction shape at p = 14, 16, 17,
+ *           candidate 0 otherwise.
  *   HIP:    candidate 0 is the VALU kernel with q, u, v, w staged in LDS,
  *           1 the VALU kernel with v, w in registers (half the LDS);
  *           candidates 2.. are the FP64 MFMA kernel's shapes as for HIP_MP,
- *           limited to 160 KiB of LDS; above p = 15 only the MFMA shapes.
+ *           limited to 160 KiB of LDS; above p = 15 only the MFMA shapes
+ *           (one element all at once up to p = 17, direction by direction).
  *           Defaults: the register VALU kernel at p = 4 and 6..13, the
  *           direction-by-direction MFMA shape (the last candidate) at
- *           p = 14, 15, the staged VALU kernel otherwise.
+ *           p = 14..19, the staged VALU kernel otherwise.
  * dgk_tile_select sets the process-wide choice for that form and order
  * (-1 restores the default); it holds for every solver that has not been
  * autotuned.  Launches with a given flux (dgk_cal_dqdt) or a stored,

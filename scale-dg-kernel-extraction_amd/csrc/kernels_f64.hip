@@ -742,15 +742,18 @@ unsigned grid1d(int64_t n, int threads) {
 constexpr int kValuShapes = 2;
 // Default per Nq, from whole-step timings of every candidate on MI355X at
 // about 1.7e7 DOF: the register VALU kernel at p = 4 and 6..13, the
-// direction-by-direction MFMA kernel (the last candidate) at p = 14, 15, the
+// direction-by-direction MFMA kernel (the last candidate) at p = 14..19, the
 // staged VALU kernel else (p = 3: 158 -> 153 us per launch once both read a
-// launch-wide geometry record through the scalar cache).
+// launch-wide geometry record through the scalar cache).  At p = 16, 17 the
+// one-element all-at-once MFMA kernel is candidate 0 (313 and 260 us per
+// launch against 276 and 249 direction by direction: at Nq = 17 its GEMMs
+// take two 16-wide column tiles).
 constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
-                                         0, 0, 0, 0};
+                                         1, 1, 0, 0};
 // Selected FP64 tile shape per Nq (see stage_f64_shapes); atomic because a
 // serving worker may launch while another thread selects.
 std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1,
-                                            3, 3, 0, 0, 0, 0};
+                                            3, 3, 1, 1, 0, 0};
 
 bool stage_f64_candidate_runs(int Nq, int cand, const StageArgs &a) {
   if (Nq < kMinNq || Nq > kMaxNqValu) return true;

@@ -135,9 +135,11 @@ struct MP {
 
 // Candidate kinds per order: for Nq^2 <= 64 one, two or four wavefronts of
 // whole elements, then one wavefront direction by direction; above, one
-// element, two elements, and one element direction by direction.  Kinds
-// whose LDS exceeds a CU's 160 KiB are dropped; the remaining ones are the
-// candidates 0, 1, ... (per-order defaults in kMpDefault below).
+// element, two elements, and one element direction by direction; above
+// p = 15 one element, all at once or direction by direction.  Kinds whose
+// LDS exceeds a CU's 160 KiB are dropped (all at once above p = 15: FP64 up
+// to p = 17, mixed up to p = 19); the remaining ones are the candidates 0,
+// 1, ... (per-order defaults in kMpDefault below).
 constexpr int kKinds = 4;
 template <int NQ>
 constexpr int kind_epb(int k) {
@@ -156,7 +158,7 @@ constexpr bool kind_seq(int k) {
 template <class P, int NQ>
 constexpr bool kind_ok(int k) {
   if (NQ * NQ > 64 && k == 3) return false;
-  if (NQ > kMaxNqValu && k != 2) return false;  // above p = 15: direction by direction only
+  if (NQ > kMaxNqValu && k == 1) return false;  // above p = 15: one element per workgroup
   return k == 0   ? MP<P, NQ, kind_epb<NQ>(0), kind_seq<NQ>(0)>::LDS_BYTES <= 160 * 1024
          : k == 1 ? MP<P, NQ, kind_epb<NQ>(1), kind_seq<NQ>(1)>::LDS_BYTES <= 160 * 1024
          : k == 2 ? MP<P, NQ, kind_epb<NQ>(2), kind_seq<NQ>(2)>::LDS_BYTES <= 160 * 1024
@@ -990,12 +992,14 @@ struct Cand {
 // candidate, with a launch-wide geometry record read through the scalar
 // cache): the two-wavefront shape at p = 1, 4, the four-wavefront shape at
 // p = 5 (136 -> 126 us per launch), the direction-by-direction shape at
-// p = 14, the one-wavefront shape else (p = 2: 178 -> 174 us, p = 3:
-// 133 -> 121 us against two wavefronts).
+// p = 14, 16, 17 (p = 16: 203 -> 157 us, p = 17: 163 -> 138 us against one
+// element all at once), the one-wavefront shape else (p = 2: 178 -> 174 us,
+// p = 3: 133 -> 121 us against two wavefronts; p = 18: 165 -> 160 us,
+// p = 19: 193 -> 177 us against direction by direction).
 constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 1, 0, 0, 1, 2, 0, 0, 0, 0,
-                                        0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
+                                        0, 0, 0, 0, 2, 0, 1, 1, 0, 0};
 std::atomic<int> g_mp_shape[kMaxNq + 1] = {0, 0, 1, 0, 0, 1, 2, 0, 0, 0, 0,
-                                           0, 0, 0, 0, 2, 0, 0, 0, 0, 0};
+                                           0, 0, 0, 0, 2, 0, 1, 1, 0, 0};
 
 template <class P, int NQ>
 int prepare_nq_all() {

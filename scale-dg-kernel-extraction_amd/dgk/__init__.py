@@ -188,7 +188,10 @@ def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True,
         mfma = mfma[:-1]
     mc = cand - nvalu
     epb = mfma[mc]
-    seq = nq > 16 or (mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or epb == 1))
+    # the direction-by-direction shape is the last one; above p = 15 it is the
+    # only one where the all-at-once shape's LDS does not fit
+    seq = (nq > 16 and len(mfma) == 1) or (
+        mc == len(mfma) - 1 and len(mfma) > 1 and (nq * nq <= 64 or epb == 1))
     prec = "PrecF64" if evaltype == EVAL_HIP else "PrecMixed"
     # the direction-by-direction shapes and the mixed one-wavefront shapes
     # test the face normals at run time (instantiated as the general form)

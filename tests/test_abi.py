@@ -190,12 +190,27 @@ def test_tile_shapes_host_side():
 
 
 def test_shapes_above_the_operator_files():
-    # p = 16..19 run on the direction-by-direction MFMA kernels only
-    for p in (16, 19):
+    # p = 16..19: one element per workgroup on the MFMA kernels, all at once
+    # where its three LDS areas fit (FP64 up to p = 17), then direction by
+    # direction
+    for p in (16, 17):
+        assert dgk.tile_shapes("HIP", p) == [1, 1]
+        assert dgk.tile_selected("HIP", p) == 1
+    for p in (18, 19):
         assert dgk.tile_shapes("HIP", p) == [1]
-        assert dgk.tile_shapes("HIP_MP", p) == [1]
         assert dgk.tile_selected("HIP", p) == 0
-    assert dgk.kernel_name("HIP", 17, 0) == "stage_mfma<PrecF64, 18, 1, true, false, false, false>"
+    for p in (16, 17, 18, 19):
+        assert dgk.tile_shapes("HIP_MP", p) == [1, 1]
+    # mixed: direction by direction at p = 16, 17, all at once at p = 18, 19
+    assert [dgk.tile_selected("HIP_MP", p) for p in (16, 17, 18, 19)] == [1, 1, 0, 0]
+    assert dgk.kernel_name("HIP", 16, 0) == "stage_mfma<PrecF64, 17, 1, false, true, false, false>"
+    assert dgk.kernel_name("HIP", 16, 1) == "stage_mfma<PrecF64, 17, 1, true, false, false, false>"
+    assert dgk.kernel_name("HIP", 17, 1) == "stage_mfma<PrecF64, 18, 1, true, false, false, false>"
+    assert dgk.kernel_name("HIP", 19, 0) == "stage_mfma<PrecF64, 20, 1, true, false, false, false>"
+    assert dgk.kernel_name("HIP_MP", 19, 0) == (
+        "stage_mfma<PrecMixed, 20, 1, false, true, false, false>")
+    assert dgk.kernel_name("HIP_MP", 19, 1) == (
+        "stage_mfma<PrecMixed, 20, 1, true, false, false, false>")
     assert dgk.kernel_name("HIP", 7, 0) == "stage_f64<8, false, true, false, false>"
     assert dgk.kernel_name("HIP", 7, 1) == "stage_f64<8, true, true, false, false>"
     assert dgk.kernel_name("HIP", 7, 1, axis_aligned=False) == "stage_f64<8, true, false, false, false>"

@@ -99,7 +99,7 @@ def test_fp64_stored_lift_kernel_for_non_separable_lift():
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("p", range(1, 16))
+@pytest.mark.parametrize("p", range(1, 20))
 def test_mp_tile_shapes_give_identical_results(p):
     """Every tile shape (elements per workgroup) gives the same bits, also with
     a partial last workgroup (30 elements)."""
