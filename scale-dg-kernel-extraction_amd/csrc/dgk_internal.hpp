@@ -95,16 +95,17 @@ __device__ __forceinline__ void nbr_entries(const int32_t *__restrict__ vmapP,
   }
 }
 // Element of launch slot `slot` (< Ne): the processing order, walked
-// forwards or backwards.  Without an order array, zplane > 0 (planes of
-// zplane elements, a multiple of 8 of them) gives the XCD that runs slots
-// [x Ne/8, (x+1) Ne/8) the planes x, x+8, x+16, ...
+// forwards or backwards.  Without an order array, zgroup = G > 0 (Ne a
+// multiple of 8 and Ne/8 a multiple of G) deals the element numbering to the
+// XCDs in groups of G consecutive elements: the XCD that runs slots
+// [x Ne/8, (x+1) Ne/8) takes the groups x, x+8, x+16, ...
 __device__ __forceinline__ int64_t slot_element(int64_t slot, int64_t Ne,
                                                 const int32_t *__restrict__ order, int reverse,
-                                                int64_t zplane) {
+                                                int64_t zgroup) {
   if (reverse) slot = Ne - 1 - slot;
   if (order) return (int64_t)order[slot];
-  if (zplane > 0) {
-    const uint32_t per = (uint32_t)(Ne >> 3), P = (uint32_t)zplane, s = (uint32_t)slot;
+  if (zgroup > 0) {
+    const uint32_t per = (uint32_t)(Ne >> 3), P = (uint32_t)zgroup, s = (uint32_t)slot;
     const uint32_t x = s / per, t = s - x * per, r = t / P;
     return (int64_t)(x + 8u * r) * P + (t - r * P);
   }
@@ -131,7 +132,7 @@ struct MeshTables {
   std::vector<double> D;          // Nq*Nq column-major
   std::vector<double> Lift;       // Nq^3*6 column-major
   std::vector<int32_t> order;     // element processing order (empty: identity)
-  int64_t zplane = 0;             // > 0: z-plane-interleaved order of planes of this size
+  int64_t zgroup = 0;             // > 0: groups of this many elements dealt to the XCDs
   bool axis_aligned = false;      // every face normal lies along its face axis
 };
 
@@ -151,14 +152,18 @@ std::vector<int32_t> brick_order(int NeX, int NeY, int NeZ, int bx, int by, int 
 // Element processing order.  The stage kernels deal consecutive slots of a
 // launch to one XCD (an eighth of the slots each).  For a mesh numbered
 // plane by plane along z with periodic +z neighbours e -> (e + P) mod Ne
-// (the reference's numbering, mod_mesh.f90:491) and a multiple of 8 planes,
-// the default order gives XCD x the planes x, x+8, x+16, ...: adjacent planes
-// then run at the same time on neighbouring XCDs, and the face traces they
-// share are fetched from HBM once while still in the memory-side cache.  Only
-// the assignment of elements to workgroups changes, never the arithmetic.
-// DGK_ORDER=identity keeps the element numbering.
+// (the reference's numbering, mod_mesh.f90:491) the default order deals the
+// numbering to the XCDs in groups (xcd_group): an eighth of a plane where P
+// is a multiple of 8, so that XCD x walks group x of every plane in turn (its
+// z-neighbours come round again on the same XCD, and the 8 XCDs stream
+// neighbouring addresses at any time); else whole planes where their count
+// is a multiple of 8 (XCD x takes the planes x, x+8, x+16, ...).  Only the
+// assignment of elements to workgroups changes, never the arithmetic.
+// DGK_ORDER=identity keeps the element numbering; DGK_ORDER=zgroup:G deals
+// groups of G elements (Ne/8 a multiple of G; else the numbering is kept).
 int64_t z_plane_size(const MeshTables &t);  // P, or 0 when not of that form
-void default_order(MeshTables &t);           // sets zplane (the kernels compute the order)
+int64_t xcd_group(int64_t P, int64_t nplanes);  // group size for planes of P, or 0
+void default_order(MeshTables &t);           // sets zgroup (the kernels compute the order)
 
 int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
                                 int64_t NhaloNode, const double *D1D,
@@ -222,7 +227,7 @@ struct StageArgs {
   const uint32_t *geoid;  // nullptr when one record
   const int32_t *order;   // processing order of elements (nullptr: identity)
   int reverse;            // walk the order backwards
-  int64_t zplane;         // order == nullptr and > 0: z-plane-interleaved order
+  int64_t zgroup;         // order == nullptr and > 0: XCD-interleaved groups of elements
   int64_t Ne;
   double ca, cb, dt;      // RK stage: out = ca*q0 + cb*(q + dt*tend)
   int mode;               // 0 = tendency, 1 = RK stage
@@ -248,7 +253,7 @@ struct StageArgsMP {
   const uint32_t *geoid;
   const int32_t *order;
   int reverse;
-  int64_t zplane;
+  int64_t zgroup;
   int64_t Ne;
   double ca, cb, dt;
   int mode;

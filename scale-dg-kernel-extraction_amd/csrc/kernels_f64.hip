@@ -409,7 +409,7 @@ stage_f64_lift(const StageArgs P) {
   const int i = ij % NQ, j = ij / NQ;
   const int64_t slot = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB + el;
   const bool active = slot < P.Ne;
-  const int64_t e = active ? slot_element(slot, P.Ne, P.order, P.reverse, P.zplane) : 0;
+  const int64_t e = active ? slot_element(slot, P.Ne, P.order, P.reverse, P.zgroup) : 0;
 
   const double *g = P.geo + (active && P.geoid ? (int64_t)P.geoid[e] * kGeoStride : 0);
   const int64_t base = e * NP + ij;
@@ -567,7 +567,7 @@ int launch_form_valu(const StageArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((stage_f64<NQ, VREG, AXAL, CVEL, GU>), dim3((unsigned)nblk),
                      dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out,
                      a.flux_in, a.nbr, a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca,
-                     a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zplane);
+                     a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zgroup);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }

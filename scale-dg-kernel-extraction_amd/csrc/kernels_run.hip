@@ -289,10 +289,10 @@ int launch_run(const StageArgs &a, hipStream_t s) {
   using C = Run<NQ>;
   const int64_t ntile = (a.Ne + C::T - 1) / C::T;
   if (ntile > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  // z-plane interleave at tile granularity when planes hold whole tiles
+  // the XCD-interleaved order at tile granularity when groups hold whole tiles
   int64_t zpt = 0;
-  if (a.zplane > 0 && a.zplane % C::T == 0) {
-    const int64_t P = a.zplane / C::T;
+  if (a.zgroup > 0 && a.zgroup % C::T == 0) {
+    const int64_t P = a.zgroup / C::T;
     if (ntile % (8 * P) == 0) zpt = P;
   }
   hipLaunchKernelGGL((stage_run<NQ, AXAL>), dim3((unsigned)ntile), dim3(C::THREADS), 0, s, a.q,

@@ -117,16 +117,27 @@ int64_t z_plane_size(const MeshTables &t) {
   return P;
 }
 
+int64_t xcd_group(int64_t P, int64_t nplanes) {
+  if (P <= 0 || nplanes <= 0) return 0;
+  if (P % 8 == 0) return P / 8;
+  // otherwise whole planes, where they split evenly over the XCDs (else the
+  // XCDs' slot ranges cut across the plane groups: 9% slower at 21^3)
+  return nplanes % 8 == 0 ? P : 0;
+}
+
 void default_order(MeshTables &t) {
   t.order.clear();
-  t.zplane = 0;
+  t.zgroup = 0;
   const char *env = std::getenv("DGK_ORDER");
   if (env && std::string(env) == "identity") return;
-  // only where the planes split evenly over the XCDs: otherwise the XCDs'
-  // slot ranges cut across the plane groups (measured 9% slower at 21^3)
   const int64_t P = z_plane_size(t);
-  if (P == 0 || (t.Ne / P) % 8 != 0) return;
-  t.zplane = P;  // the kernels compute this order; no array
+  if (P == 0) return;
+  long g = 0;
+  if (env && std::sscanf(env, "zgroup:%ld", &g) == 1) {  // a group size of choice
+    if (g > 0 && t.Ne % 8 == 0 && (t.Ne / 8) % g == 0) t.zgroup = g;
+    return;
+  }
+  t.zgroup = xcd_group(P, t.Ne / P);  // the kernels compute this order; no array
 }
 
 int load_operators(const char *dir, int p, std::vector<double> &x1D,
@@ -199,7 +210,7 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
       bz > 0)
   {
     t.order = brick_order(NeX, NeY, NeZ, bx, by, bz);
-    t.zplane = 0;
+    t.zgroup = 0;
   }
   else
     default_order(t);
@@ -255,7 +266,7 @@ int build_tables_batch(int p, const double *D1D, const double *Lift_mat,
         }
   }
   t.order.clear();
-  t.zplane = 0;
+  t.zgroup = 0;
   t.axis_aligned = true;
   return DGK_OK;
 }

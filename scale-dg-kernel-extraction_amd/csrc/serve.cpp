@@ -312,12 +312,12 @@ int start(dgk_server *sv, Req *r, int64_t off) {
   s.tile = mp ? stage_mp_selected(r->Nq) : stage_f64_selected(r->Nq);
   s.alternate = alternate_launches();
   s.launches = 0;
-  // the z-plane-interleaved order of boxes with a multiple of 8 planes
-  // (box_nbr numbers elements plane by plane, x fastest)
+  // the XCD-interleaved order of plane-numbered meshes (box_nbr numbers
+  // elements plane by plane, x fastest)
   {
     const char *env = std::getenv("DGK_ORDER");
     const bool keep = env && std::string(env) == "identity";
-    s.zplane = (!keep && r->r.ne[2] % 8 == 0) ? (int64_t)r->r.ne[0] * r->r.ne[1] : 0;
+    s.zgroup = keep ? 0 : xcd_group((int64_t)r->r.ne[0] * r->r.ne[1], r->r.ne[2]);
   }
   s.cur = 0;
 

@@ -257,29 +257,42 @@ def test_run_kernel_uniform_tilted_normals(p):
     assert np.array_equal(got[run][0], got[0][0]) and np.array_equal(got[run][1], got[0][1])
 
 
+ORDER_MESHES = {
+    # plane of 64 elements: the default deals eighths of a plane (one row of 8)
+    # to the XCDs; zgroup:64 is whole planes, zgroup:16 two rows
+    (8, 8, 16): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"},
+                 {"DGK_ORDER": "identity", "DGK_ALT": "0"}, {"DGK_ORDER": "brick:2,2,4"},
+                 {"DGK_ORDER": "zgroup:64"}, {"DGK_ORDER": "zgroup:16"}),
+    # plane of 24 elements, 5 planes: groups of 3 elements (an eighth of a plane)
+    (4, 6, 5): ({}, {"DGK_ORDER": "identity"}, {"DGK_ORDER": "zgroup:5"}),
+    # plane of 15 elements, 8 planes: whole planes
+    (3, 5, 8): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"}),
+}
+
+
+@pytest.mark.parametrize("ne", list(ORDER_MESHES))
 @pytest.mark.parametrize("form", ["HIP", "HIP_MP"])
-def test_processing_order_does_not_change_results(form, monkeypatch):
-    # 8 x 8 x 16 elements: plane-numbered with a multiple of 8 planes, so the
-    # default walks every eighth z-plane per XCD and alternates direction;
-    # the element numbering and a single direction must give the same bits,
-    # through both the structured and the reference-array constructors
+def test_processing_order_does_not_change_results(form, ne, monkeypatch):
+    # plane-numbered periodic meshes, so the default deals groups of elements
+    # to the XCDs and alternates direction; the element numbering, other
+    # group sizes and a single direction must give the same bits, through
+    # both the structured and the reference-array constructors
     p, nstep = 3, 6
     x1D = inputs.lgl_nodes(p)
-    X, Y, Z = inputs.node_coords(8, 8, 16, x1D)
+    X, Y, Z = inputs.node_coords(*ne, x1D)
     q = inputs.gaussian_blob(X, Y, Z, (0.4, 0.55, 0.3), 0.12)
     vel = (0.8, -0.45, 0.6)
     u, v, w = (np.full_like(q, c) for c in vel)
-    dt = inputs.cfl_dt(x1D, (8, 8, 16), vel)
-    m = oracle.Mesh(8, 8, 16, p)
+    dt = inputs.cfl_dt(x1D, ne, vel)
+    m = oracle.Mesh(*ne, p)
     runs = {}
-    for env in ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"},
-                {"DGK_ORDER": "identity", "DGK_ALT": "0"}, {"DGK_ORDER": "brick:2,2,4"}):
+    for env in ORDER_MESHES[ne]:
         monkeypatch.delenv("DGK_ORDER", raising=False)
         monkeypatch.delenv("DGK_ALT", raising=False)
         for k, val in env.items():
             monkeypatch.setenv(k, val)
         for how in ("structured", "arrays"):
-            s = (dgk.Solver.structured(form, 8, 8, 16, p) if how == "structured"
+            s = (dgk.Solver.structured(form, *ne, p) if how == "structured"
                  else solver_from_mesh(m, form))
             with s:
                 s.set_fields(q, u, v, w)
