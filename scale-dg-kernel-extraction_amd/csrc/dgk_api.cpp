@@ -158,7 +158,7 @@ StageArgs stage_args(dgk_solver *s, const double *qin, const double *q0,
   a.q = qin; a.q0 = q0; a.u = s->u; a.v = s->v; a.w = s->w; a.out = out;
   a.flux_in = nullptr; a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D; a.Lift = s->Lift;
   a.ell = s->ell; a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = 0;
-  a.zgroup = s->zgroup; a.Ne = s->Ne;
+  a.zperiod = s->zperiod; a.Ne = s->Ne;
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
   a.dt_geo = 0;
   a.tile = s->tile;
@@ -176,7 +176,7 @@ StageArgsMP stage_args_mp(dgk_solver *s, const double *qin, const double *q0,
   a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
   a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
   a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = 0;
-  a.zgroup = s->zgroup; a.Ne = s->Ne;
+  a.zperiod = s->zperiod; a.Ne = s->Ne;
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;
   a.dt_geo = 0;
   a.tile = s->tile;
@@ -310,7 +310,7 @@ int create_from_tables(int evaltype, MeshTables &t, dgk_solver **out) {
   // last, which are still in the memory-side cache (DGK_ALT=0: always
   // forwards).  The arithmetic of an element does not depend on when it runs.
   s->alternate = alternate_launches();
-  s->zgroup = t.zgroup;
+  s->zperiod = t.zperiod;
   if (!t.geoid.empty()) {
     const int T = run_elements(t.Nq);  // the mixed run kernel's runs are the same
     for (int64_t e = 0; T > 0 && e < t.Ne; ++e)

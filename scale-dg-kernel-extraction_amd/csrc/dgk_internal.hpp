@@ -95,19 +95,27 @@ __device__ __forceinline__ void nbr_entries(const int32_t *__restrict__ vmapP,
   }
 }
 // Element of launch slot `slot` (< Ne): the processing order, walked
-// forwards or backwards.  Without an order array, zgroup = G > 0 (Ne a
-// multiple of 8 and Ne/8 a multiple of G) deals the element numbering to the
-// XCDs in groups of G consecutive elements: the XCD that runs slots
-// [x Ne/8, (x+1) Ne/8) takes the groups x, x+8, x+16, ...
+// forwards or backwards.  Without an order array, zperiod = Q > 0 (8 <= Q,
+// Ne a multiple of Q) deals every Q consecutive elements of the numbering to
+// the 8 XCDs in eighths: with n = Ne / Q periods and o_x = floor(x Q / 8),
+// the slots [n o_x, n o_{x+1}), which the XCD x runs, are the elements
+// [o_x, o_{x+1}) of every period, period after period.  Q = P (planes of P
+// elements) gives XCD x the eighth x of every plane; Q = 8 G deals groups of
+// G elements round-robin.
 __device__ __forceinline__ int64_t slot_element(int64_t slot, int64_t Ne,
                                                 const int32_t *__restrict__ order, int reverse,
-                                                int64_t zgroup) {
+                                                int64_t zperiod) {
   if (reverse) slot = Ne - 1 - slot;
   if (order) return (int64_t)order[slot];
-  if (zgroup > 0) {
-    const uint32_t per = (uint32_t)(Ne >> 3), P = (uint32_t)zgroup, s = (uint32_t)slot;
-    const uint32_t x = s / per, t = s - x * per, r = t / P;
-    return (int64_t)(x + 8u * r) * P + (t - r * P);
+  if (zperiod > 0) {
+    const uint32_t Q = (uint32_t)zperiod, n = (uint32_t)Ne / Q, s = (uint32_t)slot;
+    uint32_t x = 0;
+#pragma unroll
+    for (uint32_t j = 1; j < 8; ++j) x += s >= n * (uint32_t)(((uint64_t)j * Q) >> 3) ? 1u : 0u;
+    const uint32_t o = (uint32_t)(((uint64_t)x * Q) >> 3);
+    const uint32_t g = (uint32_t)(((uint64_t)(x + 1) * Q) >> 3) - o;
+    const uint32_t t = s - n * o, k = t / g;
+    return (int64_t)k * Q + o + (t - k * g);
   }
   return slot;
 }
@@ -132,7 +140,7 @@ struct MeshTables {
   std::vector<double> D;          // Nq*Nq column-major
   std::vector<double> Lift;       // Nq^3*6 column-major
   std::vector<int32_t> order;     // element processing order (empty: identity)
-  int64_t zgroup = 0;             // > 0: groups of this many elements dealt to the XCDs
+  int64_t zperiod = 0;            // > 0: period of the XCD-interleaved order (slot_element)
   bool axis_aligned = false;      // every face normal lies along its face axis
 };
 
@@ -152,18 +160,16 @@ std::vector<int32_t> brick_order(int NeX, int NeY, int NeZ, int bx, int by, int 
 // Element processing order.  The stage kernels deal consecutive slots of a
 // launch to one XCD (an eighth of the slots each).  For a mesh numbered
 // plane by plane along z with periodic +z neighbours e -> (e + P) mod Ne
-// (the reference's numbering, mod_mesh.f90:491) the default order deals the
-// numbering to the XCDs in groups (xcd_group): an eighth of a plane where P
-// is a multiple of 8, so that XCD x walks group x of every plane in turn (its
-// z-neighbours come round again on the same XCD, and the 8 XCDs stream
-// neighbouring addresses at any time); else whole planes where their count
-// is a multiple of 8 (XCD x takes the planes x, x+8, x+16, ...).  Only the
+// (the reference's numbering, mod_mesh.f90:491) and P >= 8 the default order
+// is slot_element's with period P (xcd_period): XCD x walks the eighth x of
+// every plane in turn, so its z-neighbours come round again on the same XCD
+// and the 8 XCDs stream neighbouring addresses at any time.  Only the
 // assignment of elements to workgroups changes, never the arithmetic.
-// DGK_ORDER=identity keeps the element numbering; DGK_ORDER=zgroup:G deals
-// groups of G elements (Ne/8 a multiple of G; else the numbering is kept).
+// DGK_ORDER=identity keeps the element numbering; DGK_ORDER=zperiod:Q takes
+// the period Q instead (8 <= Q, Ne a multiple of Q; else the numbering).
 int64_t z_plane_size(const MeshTables &t);  // P, or 0 when not of that form
-int64_t xcd_group(int64_t P, int64_t nplanes);  // group size for planes of P, or 0
-void default_order(MeshTables &t);           // sets zgroup (the kernels compute the order)
+int64_t xcd_period(int64_t P);               // the default period for planes of P, or 0
+void default_order(MeshTables &t);           // sets zperiod (the kernels compute the order)
 
 int build_tables_from_reference(int Nq, int64_t Ne, int64_t NeA,
                                 int64_t NhaloNode, const double *D1D,
@@ -227,7 +233,7 @@ struct StageArgs {
   const uint32_t *geoid;  // nullptr when one record
   const int32_t *order;   // processing order of elements (nullptr: identity)
   int reverse;            // walk the order backwards
-  int64_t zgroup;         // order == nullptr and > 0: XCD-interleaved groups of elements
+  int64_t zperiod;        // order == nullptr and > 0: period of the XCD-interleaved order
   int64_t Ne;
   double ca, cb, dt;      // RK stage: out = ca*q0 + cb*(q + dt*tend)
   int mode;               // 0 = tendency, 1 = RK stage
@@ -253,7 +259,7 @@ struct StageArgsMP {
   const uint32_t *geoid;
   const int32_t *order;
   int reverse;
-  int64_t zgroup;
+  int64_t zperiod;
   int64_t Ne;
   double ca, cb, dt;
   int mode;

@@ -409,7 +409,7 @@ stage_f64_lift(const StageArgs P) {
   const int i = ij % NQ, j = ij / NQ;
   const int64_t slot = (int64_t)xcd_block(blockIdx.x, gridDim.x) * EPB + el;
   const bool active = slot < P.Ne;
-  const int64_t e = active ? slot_element(slot, P.Ne, P.order, P.reverse, P.zgroup) : 0;
+  const int64_t e = active ? slot_element(slot, P.Ne, P.order, P.reverse, P.zperiod) : 0;
 
   const double *g = P.geo + (active && P.geoid ? (int64_t)P.geoid[e] * kGeoStride : 0);
   const int64_t base = e * NP + ij;
@@ -558,6 +558,14 @@ int prepare_nq() {
   return prepare_shape_valu<NQ, true, true>();
 }
 
+// The VALU kernels keep the element numbering where an XCD's eighth of a
+// period is under 96 elements (per launch at about 1.7e7 DOF: p = 9..13 with
+// 84..40 elements per eighth ran 1-4% slower in the XCD-interleaved order;
+// p = 2..8 with 903..98 elements 2-11% faster).  The MFMA kernels take the
+// interleaved order at every size.
+constexpr int64_t kValuMinEighth = 96;
+int64_t valu_period(int64_t q) { return q / 8 >= kValuMinEighth ? q : 0; }
+
 template <int NQ, bool VREG, bool AXAL, bool CVEL, bool GU>
 int launch_form_valu(const StageArgs &a, hipStream_t s) {
   using T = Staged<NQ, VREG>;
@@ -567,7 +575,8 @@ int launch_form_valu(const StageArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((stage_f64<NQ, VREG, AXAL, CVEL, GU>), dim3((unsigned)nblk),
                      dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out,
                      a.flux_in, a.nbr, a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca,
-                     a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zgroup);
+                     a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo, a.reverse,
+                     valu_period(a.zperiod));
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }

@@ -870,9 +870,9 @@ int launch_run_mp_nq(const StageArgsMP &a, hipStream_t s) {
   const int64_t ntile = (a.Ne + C::T - 1) / C::T;
   if (ntile > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
   int64_t zpt = 0;  // the XCD-interleaved order at tile granularity, as stage_run
-  if (a.zgroup > 0 && a.zgroup % C::T == 0) {
-    const int64_t P = a.zgroup / C::T;
-    if (ntile % (8 * P) == 0) zpt = P;
+  if (a.zperiod > 0 && a.zperiod % C::T == 0) {
+    const int64_t Qt = a.zperiod / C::T;
+    if (Qt >= 8 && ntile % Qt == 0) zpt = Qt;
   }
   hipLaunchKernelGGL((stage_run_mp<NQ, AXAL>), dim3((unsigned)ntile), dim3(C::THREADS), 0, s,
                      a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr, a.D, a.ell, a.geo, a.geoid, a.Ne,
@@ -956,7 +956,7 @@ int launch_kernel(const Args &a, hipStream_t s) {
   hipLaunchKernelGGL((stage_mfma<P, NQ, EPB, SEQ, AXAL, CVEL, GU>), dim3((unsigned)nblk),
                      dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out, a.nbr,
                      a.vmapP, a.D, a.ell, a.geo, a.geoid, a.order, a.Ne, a.ca, a.cb, a.dt,
-                     a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zgroup);
+                     a.mode, a.axis_aligned, a.dt_geo, a.reverse, a.zperiod);
   DGK_HIP(hipGetLastError());
   return DGK_OK;
 }

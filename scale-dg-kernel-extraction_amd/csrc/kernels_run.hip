@@ -289,11 +289,11 @@ int launch_run(const StageArgs &a, hipStream_t s) {
   using C = Run<NQ>;
   const int64_t ntile = (a.Ne + C::T - 1) / C::T;
   if (ntile > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
-  // the XCD-interleaved order at tile granularity when groups hold whole tiles
+  // the XCD-interleaved order at tile granularity when periods hold whole tiles
   int64_t zpt = 0;
-  if (a.zgroup > 0 && a.zgroup % C::T == 0) {
-    const int64_t P = a.zgroup / C::T;
-    if (ntile % (8 * P) == 0) zpt = P;
+  if (a.zperiod > 0 && a.zperiod % C::T == 0) {
+    const int64_t Qt = a.zperiod / C::T;
+    if (Qt >= 8 && ntile % Qt == 0) zpt = Qt;
   }
   hipLaunchKernelGGL((stage_run<NQ, AXAL>), dim3((unsigned)ntile), dim3(C::THREADS), 0, s, a.q,
                      a.q0, a.u, a.v, a.w, a.out, a.nbr, a.D, a.ell, a.geo, a.geoid, a.Ne, a.ca,

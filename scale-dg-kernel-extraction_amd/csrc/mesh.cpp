@@ -117,27 +117,21 @@ int64_t z_plane_size(const MeshTables &t) {
   return P;
 }
 
-int64_t xcd_group(int64_t P, int64_t nplanes) {
-  if (P <= 0 || nplanes <= 0) return 0;
-  if (P % 8 == 0) return P / 8;
-  // otherwise whole planes, where they split evenly over the XCDs (else the
-  // XCDs' slot ranges cut across the plane groups: 9% slower at 21^3)
-  return nplanes % 8 == 0 ? P : 0;
-}
+int64_t xcd_period(int64_t P) { return P >= 8 ? P : 0; }
 
 void default_order(MeshTables &t) {
   t.order.clear();
-  t.zgroup = 0;
+  t.zperiod = 0;
   const char *env = std::getenv("DGK_ORDER");
   if (env && std::string(env) == "identity") return;
   const int64_t P = z_plane_size(t);
   if (P == 0) return;
-  long g = 0;
-  if (env && std::sscanf(env, "zgroup:%ld", &g) == 1) {  // a group size of choice
-    if (g > 0 && t.Ne % 8 == 0 && (t.Ne / 8) % g == 0) t.zgroup = g;
+  long q = 0;
+  if (env && std::sscanf(env, "zperiod:%ld", &q) == 1) {  // a period of choice
+    if (q >= 8 && q <= t.Ne && t.Ne % q == 0) t.zperiod = q;
     return;
   }
-  t.zgroup = xcd_group(P, t.Ne / P);  // the kernels compute this order; no array
+  t.zperiod = xcd_period(P);  // the kernels compute this order; no array
 }
 
 int load_operators(const char *dir, int p, std::vector<double> &x1D,
@@ -210,7 +204,7 @@ int build_tables_structured(int NeX, int NeY, int NeZ, int p,
       bz > 0)
   {
     t.order = brick_order(NeX, NeY, NeZ, bx, by, bz);
-    t.zgroup = 0;
+    t.zperiod = 0;
   }
   else
     default_order(t);
@@ -266,7 +260,7 @@ int build_tables_batch(int p, const double *D1D, const double *Lift_mat,
         }
   }
   t.order.clear();
-  t.zgroup = 0;
+  t.zperiod = 0;
   t.axis_aligned = true;
   return DGK_OK;
 }

@@ -317,7 +317,7 @@ int start(dgk_server *sv, Req *r, int64_t off) {
   {
     const char *env = std::getenv("DGK_ORDER");
     const bool keep = env && std::string(env) == "identity";
-    s.zgroup = keep ? 0 : xcd_group((int64_t)r->r.ne[0] * r->r.ne[1], r->r.ne[2]);
+    s.zperiod = keep ? 0 : xcd_period((int64_t)r->r.ne[0] * r->r.ne[1]);
   }
   s.cur = 0;
 

@@ -37,7 +37,7 @@ struct dgk_solver {
   double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
   int32_t *order = nullptr;  // element processing order
   bool alternate = false;    // consecutive stage launches walk the order both ways
-  int64_t zgroup = 0;        // XCD-interleaved order (groups of this many elements)
+  int64_t zperiod = 0;       // period of the XCD-interleaved order (slot_element)
   uint64_t launches = 0;     // stage launches so far (picks the direction)
   int axis_aligned = 0;
   int tile = -1;             // tile-shape candidate chosen for this mesh (-1: process default)

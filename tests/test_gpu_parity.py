@@ -258,25 +258,30 @@ def test_run_kernel_uniform_tilted_normals(p):
 
 
 ORDER_MESHES = {
-    # plane of 64 elements: the default deals eighths of a plane (one row of 8)
-    # to the XCDs; zgroup:64 is whole planes, zgroup:16 two rows
+    # plane of 64 elements: the default period 64 gives each XCD one row of 8
+    # per plane; zperiod:512 is whole planes, zperiod:128 two rows per turn
     (8, 8, 16): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"},
                  {"DGK_ORDER": "identity", "DGK_ALT": "0"}, {"DGK_ORDER": "brick:2,2,4"},
-                 {"DGK_ORDER": "zgroup:64"}, {"DGK_ORDER": "zgroup:16"}),
-    # plane of 24 elements, 5 planes: groups of 3 elements (an eighth of a plane)
-    (4, 6, 5): ({}, {"DGK_ORDER": "identity"}, {"DGK_ORDER": "zgroup:5"}),
-    # plane of 15 elements, 8 planes: whole planes
-    (3, 5, 8): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"}),
+                 {"DGK_ORDER": "zperiod:512"}, {"DGK_ORDER": "zperiod:128"}),
+    # plane of 24 elements, 5 planes: eighths of 3 elements; groups of 5
+    (4, 6, 5): ({}, {"DGK_ORDER": "identity"}, {"DGK_ORDER": "zperiod:40"}),
+    # planes of 15 and 25 elements: uneven eighths (1 or 2, 3 or 4 elements);
+    # zperiod:120 deals whole planes of 15
+    (3, 5, 8): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"}, {"DGK_ORDER": "zperiod:120"}),
+    (5, 5, 3): ({}, {"DGK_ORDER": "identity"}),
+    # eighths of 128 elements: the FP64 VALU kernels take the interleaved
+    # order too (below 96 elements per eighth they keep the numbering)
+    (32, 32, 2): ({}, {"DGK_ORDER": "identity"}, {"DGK_ORDER": "zperiod:2048"}),
 }
 
 
 @pytest.mark.parametrize("ne", list(ORDER_MESHES))
 @pytest.mark.parametrize("form", ["HIP", "HIP_MP"])
 def test_processing_order_does_not_change_results(form, ne, monkeypatch):
-    # plane-numbered periodic meshes, so the default deals groups of elements
-    # to the XCDs and alternates direction; the element numbering, other
-    # group sizes and a single direction must give the same bits, through
-    # both the structured and the reference-array constructors
+    # plane-numbered periodic meshes, so the default deals an eighth of every
+    # plane to each XCD and alternates direction; the element numbering,
+    # other periods, a brick order and a single direction must give the same
+    # bits, through both the structured and the reference-array constructors
     p, nstep = 3, 6
     x1D = inputs.lgl_nodes(p)
     X, Y, Z = inputs.node_coords(*ne, x1D)
