@@ -115,14 +115,14 @@ int dgk_mesh_build(int NeX, int NeY, int NeZ, int p, const double *x1D,
  * "HIP_MP" always uses the 1D form and rejects a Lift_mat that deviates from
  * it by more than 1e-10 relative.
  * Processing order (never the arithmetic, so never the results): a mesh
- * numbered plane by plane along z with periodic +z neighbours is dealt to
- * the GPU's 8 XCDs in groups of consecutive elements -- an eighth of a plane
- * where the plane size is a multiple of 8 (XCD x walks group x of every
- * plane), else whole planes where their count is a multiple of 8 (XCD x
- * walks the planes x, x+8, ...) -- and consecutive stage launches walk the
- * elements in opposite directions.  Environment: DGK_ORDER=identity keeps
- * the element numbering, DGK_ALT=0 keeps one direction (DGK_ORDER=zgroup:G
- * deals groups of G elements, G dividing Ne/8; DGK_ORDER=brick:bx,by,bz,
+ * numbered plane by plane along z with periodic +z neighbours and planes of
+ * at least 8 elements is dealt to the GPU's 8 XCDs in eighths of every
+ * plane (XCD x walks the eighth x of every plane in turn; the FP64 VALU
+ * kernels keep the element numbering where an eighth holds fewer than 96
+ * elements), and consecutive stage launches walk the elements in opposite
+ * directions.  Environment: DGK_ORDER=identity keeps the element numbering,
+ * DGK_ALT=0 keeps one direction (DGK_ORDER=zperiod:Q deals eighths of every
+ * Q elements instead, 8 <= Q, Q dividing Ne; DGK_ORDER=brick:bx,by,bz,
  * structured solvers only, walks bricks of elements instead). */
 int dgk_create(int evaltype, int Nq, int64_t Ne, int64_t NeA,
                int64_t NhaloNode, const double *D1D, const double *Lift_mat,
