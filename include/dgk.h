@@ -259,10 +259,14 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  *           (a third of the LDS), for p <= 7; for p >= 8 one element, two
  *           elements, and one element direction by direction; above p = 15
  *           one element, all at once and direction by direction.  Shapes
- *           are limited to 160 KiB of LDS.  Defaults: two wavefronts
- *           (candidate 1) at p = 1, 4, four wavefronts (candidate 2) at
- *           p = 5, the direction-by-direction shape at p = 14, 16, 17,
- *           candidate 0 otherwise.
+ *           are limited to 160 KiB of LDS.  At p <= 4 the next candidate
+ *           advances runs of consecutive elements, and at p <= 7 the last
+ *           is the FP64 register VALU kernel reading the FP32 velocity
+ *           fields (FP64 arithmetic: its bits differ from the MFMA shapes',
+ *           which agree with each other).  Defaults: the VALU kernel at
+ *           p = 1, two wavefronts (candidate 1) at p = 4, four wavefronts
+ *           (candidate 2) at p = 5, the direction-by-direction shape at
+ *           p = 14, 16, 17, candidate 0 otherwise.
  *   HIP:    candidate 0 is the VALU kernel with q, u, v, w staged in LDS,
  *           1 the VALU kernel with v, w in registers (half the LDS);
  *           candidates 2.. are the FP64 MFMA kernel's shapes as for HIP_MP,

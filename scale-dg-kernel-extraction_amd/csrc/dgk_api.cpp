@@ -175,6 +175,7 @@ StageArgsMP stage_args_mp(dgk_solver *s, const double *qin, const double *q0,
   StageArgsMP a;
   a.q = qin; a.q0 = q0; a.u = s->u32; a.v = s->v32; a.w = s->w32; a.out = out;
   a.nbr = s->nbr; a.vmapP = s->vmapP; a.D = s->D32; a.ell = s->ell32;
+  a.D64 = s->D; a.ell64 = s->ell;
   a.geo = s->geo; a.geoid = s->geoid; a.order = s->order; a.reverse = 0;
   a.zperiod = s->zperiod; a.Ne = s->Ne;
   a.ca = ca; a.cb = cb; a.dt = dt; a.mode = mode; a.axis_aligned = s->axis_aligned;

@@ -255,6 +255,8 @@ struct StageArgsMP {
   const int32_t *vmapP;
   const float *D;         // Nq*Nq, FP32
   const float *ell;       // 2*Nq: l-(x) then l+(x), the separable 1D Lift
+  const double *D64 = nullptr;    // the same in FP64, for the VALU kernel
+  const double *ell64 = nullptr;
   const double *geo;
   const uint32_t *geoid;
   const int32_t *order;
@@ -277,6 +279,11 @@ int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s);
 // another shape instead).
 bool stage_f64_candidate_runs(int Nq, int cand, const StageArgs &a);
 bool stage_mp_candidate_runs(int Nq, int cand, const StageArgsMP &a);
+// The mixed form's VALU kernel (FP32 velocity fields, FP64 arithmetic) at
+// p <= 7: elements per workgroup (0 where there is none), launch, set-up.
+int valu_mp_elements(int Nq);
+int launch_stage_valu_mp(int Nq, const StageArgsMP &a, hipStream_t s);
+int prepare_stage_valu_mp(int Nq);
 // Tile shapes (elements per block) of the mixed-precision stage kernel of
 // order Nq (the MFMA shapes, then at p <= 4 the run kernel): fills epb and
 // returns their number (n >= 5).  stage_mp_select picks the

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <atomic>
 #include <mutex>
+#include <type_traits>
 
 #include "dgk_internal.hpp"
 
@@ -111,22 +112,28 @@ struct Staged {
 // they equal, so the results are the same bits.
 // GU: several elements per workgroup (p <= 4) and one geometry record for the
 // whole launch (no per-element record ids).
-template <int NQ, bool VREG, bool AXAL, bool CVEL, bool GU>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, VREG>::THREADS),
-                               amdgpu_waves_per_eu(AXAL ? Staged<NQ, VREG>::MIN_WAVES_EU_AX
-                                                        : Staged<NQ, VREG>::MIN_WAVES_EU)))
-stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
-          const double *__restrict__ v_, const double *__restrict__ w_, double *out_,
-          const double *__restrict__ flux_in_, const int32_t *__restrict__ nbr_,
-          const int32_t *__restrict__ vmapP_, const double *__restrict__ D_,
-          const double *__restrict__ ell_, const double *__restrict__ geo_,
-          const uint32_t *__restrict__ geoid_, const int32_t *__restrict__ order_,
-          const int64_t Ne_, const double ca_, const double cb_, const double dt_,
-          const int mode_, const int axis_aligned_, const int dt_geo_, const int rev_,
-          const int64_t zp_) {
+// VT: the velocity fields' type.  double for the FP64 form (stage_f64);
+// float for the mixed form's VALU kernel (stage_f64_mp): the velocities are
+// stored in FP32 (4 of the 100 bytes per DOF-update of that form) and every
+// operation is FP64, so its results are those of the FP64 form with the
+// velocities rounded to FP32.  With float velocities only q is copied to LDS
+// asynchronously; each lane stages its own u column as doubles.
+template <int NQ, bool VREG, bool AXAL, bool CVEL, bool GU, typename VT>
+__device__ __forceinline__ void
+stage_valu(const double *__restrict__ q_, const double *q0_, const VT *__restrict__ u_,
+           const VT *__restrict__ v_, const VT *__restrict__ w_, double *out_,
+           const double *__restrict__ flux_in_, const int32_t *__restrict__ nbr_,
+           const int32_t *__restrict__ vmapP_, const double *__restrict__ D_,
+           const double *__restrict__ ell_, const double *__restrict__ geo_,
+           const uint32_t *__restrict__ geoid_, const int32_t *__restrict__ order_,
+           const int64_t Ne_, const double ca_, const double cb_, const double dt_,
+           const int mode_, const int axis_aligned_, const int dt_geo_, const int rev_,
+           const int64_t zp_) {
   // Read-only operands are __restrict__ kernel parameters, so wave-uniform
   // reads of them (operators, geometry, neighbour table) are scalar loads.
   static_assert(!CVEL || AXAL, "constant velocity on axis-aligned faces");
+  constexpr bool F64_VEL = std::is_same<VT, double>::value;
+  static_assert(F64_VEL || VREG, "FP32 velocities: the register form only");
   using C = Staged<NQ, VREG>;
   constexpr int NQ2 = C::NQ2, NP = C::NP, EPB = C::EPB, NFT = 6 * NQ2;
   extern __shared__ double smem[];
@@ -191,7 +198,13 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
     // ---- 1. async HBM -> LDS copy of q, u, v, w -------------------------------
     {
       const int wave = tid >> 6, lane = tid & 63;
-      const double *src[4] = {q_, u_, v_, w_};
+      const double *src[4] = {q_, nullptr, nullptr, nullptr};
+      if constexpr (F64_VEL) {
+        src[1] = u_;
+        src[2] = v_;
+        src[3] = w_;
+      }
+      constexpr int NASYNC = (CVEL || !F64_VEL) ? 1 : C::NSTAGED;
 #pragma unroll
       for (int c0 = 0; c0 < C::NCHUNK; c0 += C::WAVES) {
         const int c = c0 + wave;
@@ -205,7 +218,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
             off = ge * NP + dn - (C::ODD ? (ge & 1) : 0);
           }
 #pragma unroll
-          for (int fld = 0; fld < (CVEL ? 1 : C::NSTAGED); ++fld) {
+          for (int fld = 0; fld < NASYNC; ++fld) {
             const double *gp = src[fld] + off;
             __attribute__((address_space(3))) void *lp =
                 (__attribute__((address_space(3))) void *)(smem + fld * C::FIELD + c * C::CHUNK);
@@ -217,6 +230,15 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 #endif
           }
         }
+      }
+    }
+
+    // FP32 velocities: this lane's u column, as doubles, into the staged u
+    // field at the slots the async copy would fill
+    if constexpr (!F64_VEL && !CVEL) {
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) su[lb + ij + k * NQ2] = (double)u_[base + (int64_t)k * NQ2];
       }
     }
 
@@ -238,7 +260,7 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
           const int64_t pidx = nbr_at(f);
-          const double *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
+          const VT *vel = kFaceAxis[f] == 0 ? u_ : kFaceAxis[f] == 1 ? v_ : w_;
           qP[f] = q_[pidx];
           uP[f] = CVEL ? cvel[kFaceAxis[f]] : vel[pidx];
         }
@@ -383,6 +405,43 @@ stage_f64(const double *__restrict__ q_, const double *q0_, const double *__rest
   };
   if constexpr (GU) body(geo_);
   else body(g_lane);
+}
+
+template <int NQ, bool VREG, bool AXAL, bool CVEL, bool GU>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, VREG>::THREADS),
+                               amdgpu_waves_per_eu(AXAL ? Staged<NQ, VREG>::MIN_WAVES_EU_AX
+                                                        : Staged<NQ, VREG>::MIN_WAVES_EU)))
+stage_f64(const double *__restrict__ q_, const double *q0_, const double *__restrict__ u_,
+          const double *__restrict__ v_, const double *__restrict__ w_, double *out_,
+          const double *__restrict__ flux_in_, const int32_t *__restrict__ nbr_,
+          const int32_t *__restrict__ vmapP_, const double *__restrict__ D_,
+          const double *__restrict__ ell_, const double *__restrict__ geo_,
+          const uint32_t *__restrict__ geoid_, const int32_t *__restrict__ order_,
+          const int64_t Ne_, const double ca_, const double cb_, const double dt_,
+          const int mode_, const int axis_aligned_, const int dt_geo_, const int rev_,
+          const int64_t zp_) {
+  stage_valu<NQ, VREG, AXAL, CVEL, GU, double>(q_, q0_, u_, v_, w_, out_, flux_in_, nbr_,
+                                               vmapP_, D_, ell_, geo_, geoid_, order_, Ne_,
+                                               ca_, cb_, dt_, mode_, axis_aligned_, dt_geo_,
+                                               rev_, zp_);
+}
+
+// The mixed form's VALU kernel (register form, FP32 velocity fields).
+template <int NQ, bool AXAL, bool CVEL, bool GU>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, Staged<NQ, true>::THREADS),
+                               amdgpu_waves_per_eu(AXAL ? Staged<NQ, true>::MIN_WAVES_EU_AX
+                                                        : Staged<NQ, true>::MIN_WAVES_EU)))
+stage_f64_mp(const double *__restrict__ q_, const double *q0_, const float *__restrict__ u_,
+             const float *__restrict__ v_, const float *__restrict__ w_, double *out_,
+             const int32_t *__restrict__ nbr_, const int32_t *__restrict__ vmapP_,
+             const double *__restrict__ D_, const double *__restrict__ ell_,
+             const double *__restrict__ geo_, const uint32_t *__restrict__ geoid_,
+             const int32_t *__restrict__ order_, const int64_t Ne_, const double ca_,
+             const double cb_, const double dt_, const int mode_, const int axis_aligned_,
+             const int dt_geo_, const int rev_, const int64_t zp_) {
+  stage_valu<NQ, true, AXAL, CVEL, GU, float>(q_, q0_, u_, v_, w_, out_, nullptr, nbr_, vmapP_,
+                                              D_, ell_, geo_, geoid_, order_, Ne_, ca_, cb_,
+                                              dt_, mode_, axis_aligned_, dt_geo_, rev_, zp_);
 }
 
 // Stored-Lift kernel: Lift_mat read as stored (any Lift_mat).
@@ -607,6 +666,50 @@ int launch_nq(const StageArgs &a, hipStream_t s, bool vreg) {
   return DGK_OK;
 }
 
+template <int NQ, bool AXAL, bool CVEL, bool GU>
+int prepare_form_valu_mp() {
+  // set once per device; concurrent callers may both set it (idempotent)
+  static std::atomic<int> prepared_dev{-1};
+  int dev = 0;
+  DGK_HIP(hipGetDevice(&dev));
+  if (prepared_dev.load(std::memory_order_acquire) != dev) {
+    DGK_HIP(hipFuncSetAttribute((const void *)stage_f64_mp<NQ, AXAL, CVEL, GU>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)Staged<NQ, true>::LDS_BYTES));
+    prepared_dev.store(dev, std::memory_order_release);
+  }
+  return DGK_OK;
+}
+
+template <int NQ, bool AXAL, bool CVEL = false>
+int prepare_shape_valu_mp() {
+  DGK_TRY((prepare_form_valu_mp<NQ, AXAL, CVEL, false>()));
+  if constexpr (has_gu<NQ, true>()) DGK_TRY((prepare_form_valu_mp<NQ, AXAL, CVEL, true>()));
+  return DGK_OK;
+}
+
+template <int NQ, bool AXAL, bool CVEL, bool GU>
+int launch_form_valu_mp(const StageArgsMP &a, hipStream_t s) {
+  using T = Staged<NQ, true>;
+  const int64_t nblk = (a.Ne + T::EPB - 1) / T::EPB;
+  if (nblk > INT32_MAX) return set_error(DGK_ERR_UNSUPPORTED, "too many elements");
+  DGK_TRY((prepare_form_valu_mp<NQ, AXAL, CVEL, GU>()));
+  hipLaunchKernelGGL((stage_f64_mp<NQ, AXAL, CVEL, GU>), dim3((unsigned)nblk),
+                     dim3(T::THREADS), T::LDS_BYTES, s, a.q, a.q0, a.u, a.v, a.w, a.out,
+                     a.nbr, a.vmapP, a.D64, a.ell64, a.geo, a.geoid, a.order, a.Ne, a.ca,
+                     a.cb, a.dt, a.mode, a.axis_aligned, a.dt_geo, a.reverse,
+                     valu_period(a.zperiod));
+  DGK_HIP(hipGetLastError());
+  return DGK_OK;
+}
+
+template <int NQ, bool AXAL, bool CVEL = false>
+int launch_valu_mp(const StageArgsMP &a, hipStream_t s) {
+  if constexpr (has_gu<NQ, true>())
+    if (!a.geoid) return launch_form_valu_mp<NQ, AXAL, CVEL, true>(a, s);
+  return launch_form_valu_mp<NQ, AXAL, CVEL, false>(a, s);
+}
+
 // cal_elembnd_flux: one thread per face node (mod_advect3d_eq.f90:116-140).
 __global__ void elembnd_flux_f64(const double *__restrict__ q,
                                  const double *__restrict__ u,
@@ -813,6 +916,48 @@ int prepare_stage_f64(int Nq) {
   DGK_VALU(16)
 #undef DGK_VALU
   default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
+  }
+}
+
+int valu_mp_elements(int Nq) {
+  switch (Nq) {
+#define DGK_VALU_MP(N) \
+  case N: return Staged<N, true>::EPB;
+  DGK_VALU_MP(2) DGK_VALU_MP(3) DGK_VALU_MP(4) DGK_VALU_MP(5) DGK_VALU_MP(6) DGK_VALU_MP(7)
+  DGK_VALU_MP(8)
+#undef DGK_VALU_MP
+  default: return 0;
+  }
+}
+
+int launch_stage_valu_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
+  if (a.Ne < 1) return DGK_OK;
+  if (!a.D64 || !a.ell64)
+    return set_error(DGK_ERR_UNSUPPORTED, "the mixed VALU kernel needs the FP64 operators");
+  const bool ax = a.axis_aligned != 0;
+  switch (Nq) {
+#define DGK_VALU_MP(N)                                                     \
+  case N:                                                                  \
+    if (a.cvel && ax && !a.vmapP) return launch_valu_mp<N, true, true>(a, s); \
+    return ax ? launch_valu_mp<N, true>(a, s) : launch_valu_mp<N, false>(a, s);
+  DGK_VALU_MP(2) DGK_VALU_MP(3) DGK_VALU_MP(4) DGK_VALU_MP(5) DGK_VALU_MP(6) DGK_VALU_MP(7)
+  DGK_VALU_MP(8)
+#undef DGK_VALU_MP
+  default: return set_error(DGK_ERR_UNSUPPORTED, "no mixed VALU kernel at this order");
+  }
+}
+
+int prepare_stage_valu_mp(int Nq) {
+  switch (Nq) {
+#define DGK_VALU_MP(N)                                          \
+  case N:                                                       \
+    DGK_TRY((prepare_shape_valu_mp<N, false>()));               \
+    DGK_TRY((prepare_shape_valu_mp<N, true, true>()));          \
+    return prepare_shape_valu_mp<N, true>();
+  DGK_VALU_MP(2) DGK_VALU_MP(3) DGK_VALU_MP(4) DGK_VALU_MP(5) DGK_VALU_MP(6) DGK_VALU_MP(7)
+  DGK_VALU_MP(8)
+#undef DGK_VALU_MP
+  default: return DGK_OK;
   }
 }
 

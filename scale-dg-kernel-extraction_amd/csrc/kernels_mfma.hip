@@ -990,15 +990,17 @@ struct Cand {
 // Tile-shape candidate selected per Nq for the mixed form.  Defaults from
 // per-order timings on MI355X at about 1.7e7 DOF (whole-step runs, every
 // candidate, with a launch-wide geometry record read through the scalar
-// cache): the two-wavefront shape at p = 1, 4, the four-wavefront shape at
+// cache): the VALU kernel at p = 1 (154 -> 149 us per launch against one
+// wavefront of MFMA tiles; level at p = 2, slower above), the two-wavefront
+// shape at p = 4, the four-wavefront shape at
 // p = 5 (136 -> 126 us per launch), the direction-by-direction shape at
 // p = 14, 16, 17 (p = 16: 203 -> 157 us, p = 17: 163 -> 138 us against one
 // element all at once), the one-wavefront shape else (p = 2: 178 -> 174 us,
 // p = 3: 133 -> 121 us against two wavefronts; p = 18: 165 -> 160 us,
 // p = 19: 193 -> 177 us against direction by direction).
-constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 1, 0, 0, 1, 2, 0, 0, 0, 0,
+constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 5, 0, 0, 1, 2, 0, 0, 0, 0,
                                         0, 0, 0, 0, 2, 0, 1, 1, 0, 0};
-std::atomic<int> g_mp_shape[kMaxNq + 1] = {0, 0, 1, 0, 0, 1, 2, 0, 0, 0, 0,
+std::atomic<int> g_mp_shape[kMaxNq + 1] = {0, 0, 5, 0, 0, 1, 2, 0, 0, 0, 0,
                                            0, 0, 0, 0, 2, 0, 1, 1, 0, 0};
 
 template <class P, int NQ>
@@ -1048,26 +1050,44 @@ int shapes_of(int *epb) {
   default: return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));                   \
   }
 
+static int mp_mfma_shapes(int Nq, int *epb);
+
+// Candidates of the mixed form: the MFMA shapes, then the run kernel at
+// p <= 4, then the VALU kernel at p <= 7 (-1: none at this order).
+static int mp_run_index(int Nq) {
+  int e[kMaxTileShapes];
+  return run_mp_elements(Nq) > 0 ? mp_mfma_shapes(Nq, e) : -1;
+}
+static int mp_valu_index(int Nq) {
+  int e[kMaxTileShapes];
+  if (Nq < kMinNq || Nq > kMaxNq || valu_mp_elements(Nq) <= 0) return -1;
+  return mp_mfma_shapes(Nq, e) + (run_mp_elements(Nq) > 0 ? 1 : 0);
+}
+
 int prepare_stage_mp(int Nq) {
+  DGK_TRY(prepare_stage_valu_mp(Nq));
 #define DGK_CALL(N) prepare_nq_all<PrecMixed, N>()
   DGK_NQ_SWITCH(Nq, DGK_CALL)
 #undef DGK_CALL
 }
 
 bool stage_mp_candidate_runs(int Nq, int cand, const StageArgsMP &a) {
-  int e[kMaxTileShapes];
-  return !(run_mp_elements(Nq) > 0 && cand == stage_mp_shapes(Nq, e, kMaxTileShapes) - 1 &&
-           !run_mp_applicable(Nq, a));
+  if (cand == mp_run_index(Nq)) return run_mp_applicable(Nq, a);
+  if (cand == mp_valu_index(Nq)) return a.D64 && a.ell64;
+  return true;
 }
 
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
   int cand = a.tile;
   if (cand < 0 && Nq >= kMinNq && Nq <= kMaxNq) cand = g_mp_shape[Nq].load(std::memory_order_relaxed);
-  // the last candidate at p <= 4 is the run kernel; launches it cannot take
-  // run the first shape, with the same bits
-  int e[kMaxTileShapes];
-  if (run_mp_elements(Nq) > 0 && cand == stage_mp_shapes(Nq, e, kMaxTileShapes) - 1) {
+  // the VALU kernel (without the FP64 operators, the first shape runs) and the
+  // run kernel (launches it cannot take run the first shape, with the same bits)
+  if (cand >= 0 && cand == mp_valu_index(Nq)) {
+    if (a.D64 && a.ell64) return launch_stage_valu_mp(Nq, a, s);
+    cand = 0;
+  }
+  if (cand >= 0 && cand == mp_run_index(Nq)) {
     if (run_mp_applicable(Nq, a)) return launch_run_mp(Nq, a, s);
     cand = 0;
   }
@@ -1096,12 +1116,10 @@ static int mp_mfma_shapes(int Nq, int *epb) {
 }
 
 int stage_mp_shapes(int Nq, int *epb, int n) {
-  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < kKinds + 1) return 0;
-  const int nc = mp_mfma_shapes(Nq, epb);
-  if (nc > 0 && run_mp_elements(Nq) > 0) {
-    epb[nc] = run_mp_elements(Nq);
-    return nc + 1;
-  }
+  if (Nq < kMinNq || Nq > kMaxNq || !epb || n < kKinds + 2) return 0;
+  int nc = mp_mfma_shapes(Nq, epb);
+  if (nc > 0 && run_mp_elements(Nq) > 0) epb[nc++] = run_mp_elements(Nq);
+  if (nc > 0 && valu_mp_elements(Nq) > 0) epb[nc++] = valu_mp_elements(Nq);
   return nc;
 }
 

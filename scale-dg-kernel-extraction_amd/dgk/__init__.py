@@ -170,22 +170,24 @@ def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True,
     if isinstance(evaltype, str):
         evaltype = evaltype_id(evaltype)
     nq = p + 1
+    b = lambda x: "true" if x else "false"
     nvalu = 2 if evaltype == EVAL_HIP and nq <= 16 else 0  # FP64 VALU kernels first
     if cand < nvalu:
-        b = lambda x: "true" if x else "false"
         gu = uniform_geometry and nq * nq <= 32  # several elements per workgroup
         return (f"stage_f64<{nq}, {b(cand == 1)}, {b(axis_aligned)}, {b(constant_velocity)}, "
                 f"{b(gu)}>")
     shapes = tile_shapes(evaltype, p)
-    # the last candidate is the kernel over runs of elements: FP64 at p <= 7,
-    # mixed at p <= 4
+    # after the MFMA shapes: the kernel over runs of elements (FP64 at p <= 7,
+    # mixed at p <= 4), then the mixed form's VALU kernel (p <= 7)
     has_run = nq <= 8 if evaltype == EVAL_HIP else nq <= 5
-    if has_run and cand == len(shapes) - 1:
+    has_valu_mp = evaltype != EVAL_HIP and nq <= 8
+    if has_valu_mp and cand == len(shapes) - 1:
+        gu = uniform_geometry and nq * nq <= 32
+        return f"stage_f64_mp<{nq}, {b(axis_aligned)}, {b(constant_velocity)}, {b(gu)}>"
+    if has_run and cand == len(shapes) - 1 - int(has_valu_mp):
         name = "stage_run" if evaltype == EVAL_HIP else "stage_run_mp"
-        return f"{name}<{nq}, {'true' if axis_aligned else 'false'}>"
-    mfma = shapes[nvalu:]
-    if has_run:
-        mfma = mfma[:-1]
+        return f"{name}<{nq}, {b(axis_aligned)}>"
+    mfma = shapes[nvalu:len(shapes) - int(has_run) - int(has_valu_mp)]
     mc = cand - nvalu
     epb = mfma[mc]
     # the direction-by-direction shape is the last one; above p = 15 it is the
@@ -197,7 +199,6 @@ def kernel_name(evaltype, p: int, cand: int, axis_aligned: bool = True,
     # test the face normals at run time (instantiated as the general form)
     one_wave = epb * nq * nq <= 64
     ax_runtime = seq or (evaltype != EVAL_HIP and one_wave)
-    b = lambda x: "true" if x else "false"
     return (f"stage_mfma<{prec}, {nq}, {epb}, {b(seq)}, {b(axis_aligned and not ax_runtime)}, "
             f"{b(constant_velocity)}, {b(uniform_geometry and epb > 1)}>")
 

@@ -143,10 +143,16 @@ def test_serve_request_stream_ranges():
 
 def test_tile_shapes_host_side():
     # p <= 7: one, two, four wavefronts of elements, then one wavefront
-    # direction by direction; at p <= 4 last the run of consecutive elements
-    assert dgk.tile_shapes("HIP_MP", 7) == [1, 2, 4, 1]
-    assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16, 4, 8]
+    # direction by direction; at p <= 4 the run of consecutive elements; last
+    # the VALU kernel with FP32 velocity fields
+    assert dgk.tile_shapes("HIP_MP", 7) == [1, 2, 4, 1, 1]
+    assert dgk.tile_shapes("HIP_MP", 3) == [4, 8, 16, 4, 8, 4]
     assert dgk.kernel_name("HIP_MP", 3, 4) == "stage_run_mp<4, true>"
+    assert dgk.kernel_name("HIP_MP", 3, 5) == "stage_f64_mp<4, true, false, true>"
+    assert dgk.kernel_name("HIP_MP", 7, 4) == "stage_f64_mp<8, true, false, false>"
+    assert dgk.kernel_name("HIP_MP", 7, 3) == "stage_mfma<PrecMixed, 8, 1, true, false, false, false>"
+    assert dgk.kernel_name("HIP_MP", 8, 2) == "stage_mfma<PrecMixed, 9, 1, true, false, false, false>"
+    assert dgk.tile_shapes("HIP_MP", 8) == [1, 2, 1]
     # p >= 8: one element, two elements, one element direction by direction
     assert dgk.tile_shapes("HIP_MP", 15) == [1, 2, 1]
     # FP64: the two VALU kernels, then the FP64 MFMA kernel's shapes (two
@@ -170,14 +176,16 @@ def test_tile_shapes_host_side():
     assert dgk.tile_selected("HIP_MP", 7) == 2
     dgk.tile_select("HIP_MP", 7, -1)
     assert dgk.tile_selected("HIP_MP", 7) == 0
-    # mixed defaults: two wavefronts at p = 1, 4, four at p = 5, direction
-    # by direction at p = 14, one wavefront (or one element) else
+    # mixed defaults: the VALU kernel (last candidate) at p = 1, two
+    # wavefronts at p = 4, four at p = 5, direction by direction at p = 14,
+    # one wavefront (or one element) else
     assert [dgk.tile_selected("HIP_MP", p) for p in range(1, 16)] == (
-        [1, 0, 0, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
+        [5, 0, 0, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0])
+    assert dgk.kernel_name("HIP_MP", 1, 5) == "stage_f64_mp<2, true, false, true>"
     dgk.tile_select("HIP_MP", 1, 0)
     assert dgk.tile_selected("HIP_MP", 1) == 0
     dgk.tile_select("HIP_MP", 1, -1)
-    assert dgk.tile_selected("HIP_MP", 1) == 1
+    assert dgk.tile_selected("HIP_MP", 1) == 5
     # FP64 defaults: the register VALU kernel at p = 4 and 6..13, the
     # direction-by-direction MFMA kernel (the last candidate) at p = 14, 15,
     # the staged VALU kernel else

@@ -99,16 +99,22 @@ def test_fp64_stored_lift_kernel_for_non_separable_lift():
     assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+def mfma_candidates(p):
+    """The mixed form's MFMA and run-kernel candidates (not its VALU kernel)."""
+    return [c for c in range(len(dgk.tile_shapes("HIP_MP", p)))
+            if not dgk.kernel_name("HIP_MP", p, c).startswith("stage_f64_mp<")]
+
+
 @pytest.mark.parametrize("p", range(1, 20))
 def test_mp_tile_shapes_give_identical_results(p):
-    """Every tile shape (elements per workgroup) gives the same bits, also with
-    a partial last workgroup (30 elements)."""
+    """Every MFMA tile shape (elements per workgroup) and the run kernel give
+    the same bits, also with a partial last workgroup (30 elements)."""
     ne = (3, 2, 5) if p <= 7 else (2, 1, 3)
     q, u, v, w = inputs.random_fields(*ne, p, seed=700 + p, vel_scale=0.8)
     dt = 0.2 * inputs.cfl_dt(inputs.lgl_nodes(p), ne, (0.8, 0.8, 0.8))
     got = []
     try:
-        for cand in range(len(dgk.tile_shapes("HIP_MP", p))):
+        for cand in mfma_candidates(p):
             dgk.tile_select("HIP_MP", p, cand)
             with dgk.Solver.structured("HIP_MP", *ne, p) as s:
                 s.set_fields(q, u, v, w)
@@ -118,6 +124,38 @@ def test_mp_tile_shapes_give_identical_results(p):
         dgk.tile_select("HIP_MP", p, -1)
     for k in range(1, len(got)):
         assert np.array_equal(got[k], got[0]), k
+
+
+@pytest.mark.parametrize("p", range(1, 8))
+def test_mp_valu_kernel_is_the_fp64_register_kernel_on_fp32_velocities(p):
+    """The mixed form's VALU kernel (its last candidate at p <= 7) reads the
+    velocity fields in FP32 and computes in FP64: its results equal, bit for
+    bit, the FP64 register kernel's on the same velocities rounded to FP32,
+    for the tendency and for 3 RK steps, with a partial last workgroup."""
+    ne = (3, 2, 5)
+    q, u, v, w = inputs.random_fields(*ne, p, seed=760 + p, vel_scale=0.8)
+    dt = 0.2 * inputs.cfl_dt(inputs.lgl_nodes(p), ne, (0.8, 0.8, 0.8))
+    u32, v32, w32 = (x.astype(np.float32).astype(np.float64) for x in (u, v, w))
+    cand = len(dgk.tile_shapes("HIP_MP", p)) - 1
+    assert dgk.kernel_name("HIP_MP", p, cand).startswith("stage_f64_mp<")
+    try:
+        dgk.tile_select("HIP_MP", p, cand)
+        dgk.tile_select("HIP", p, 1)
+        with dgk.Solver.structured("HIP_MP", *ne, p) as s:
+            s.set_fields(q, u, v, w)
+            tend_mp = s.cal_tend()
+            s.step(dt, 3)
+            got = s.get_field(dgk.FIELD_Q)
+        with dgk.Solver.structured("HIP", *ne, p) as s:
+            s.set_fields(q, u32, v32, w32)
+            tend_ref = s.cal_tend()
+            s.step(dt, 3)
+            ref = s.get_field(dgk.FIELD_Q)
+    finally:
+        dgk.tile_select("HIP_MP", p, -1)
+        dgk.tile_select("HIP", p, -1)
+    assert np.array_equal(tend_mp, tend_ref)
+    assert np.array_equal(got, ref)
 
 
 def test_autotune_solver_and_batch():
@@ -138,7 +176,22 @@ def test_autotune_solver_and_batch():
     with dgk.Solver.structured("HIP_MP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
         s.set_fields(wl.q, wl.u, wl.v, wl.w)
         s.step(wl.dt, 4)
-        assert np.array_equal(s.get_field(dgk.FIELD_Q), tuned)  # shapes give the same bits
+        untuned = s.get_field(dgk.FIELD_Q)
+    # the tuned solver ran 2 steps on the default, then 2 on the best candidate,
+    # as a solver does with that candidate selected in between
+    with dgk.Solver.structured("HIP_MP", wl.NeX, wl.NeY, wl.NeZ, wl.p) as s:
+        s.set_fields(wl.q, wl.u, wl.v, wl.w)
+        s.step(wl.dt, 2)
+        try:
+            dgk.tile_select("HIP_MP", wl.p, r["best"])
+            s.step(wl.dt, 2)
+        finally:
+            dgk.tile_select("HIP_MP", wl.p, -1)
+        assert np.array_equal(s.get_field(dgk.FIELD_Q), tuned)
+    if r["best"] in mfma_candidates(wl.p):
+        assert np.array_equal(untuned, tuned)  # the MFMA shapes give the same bits
+    else:  # the VALU kernel: FP64 arithmetic on the FP32 velocities
+        assert np.linalg.norm(untuned - tuned) <= 1e-6 * np.linalg.norm(untuned)
     ens = inputs.ensemble(n=6, seed=31, ne_choices=(2, 3), p_choices=(3, 7), nstep=4)
     with dgk.Batch("HIP_MP", ens) as b, dgk.Batch("HIP_MP", ens) as ref:
         for bb in (b, ref):
@@ -156,5 +209,10 @@ def test_autotune_solver_and_batch():
             else:
                 assert x["us_per_launch"] <= 0.98 * x["us_default"]
         b.step(1)  # re-captured with the tuned shapes
-        for k in range(len(ens)):
-            assert np.array_equal(b.get_field(k), ref.get_field(k)), k
+        valu = {x["p"] for x in rec if x["candidate"] not in mfma_candidates(x["p"])}
+        for k, it in enumerate(ens):
+            got, want = b.get_field(k), ref.get_field(k)
+            if it["p"] in valu:
+                assert np.linalg.norm(got - want) <= 1e-6 * np.linalg.norm(want), k
+            else:
+                assert np.array_equal(got, want), k
