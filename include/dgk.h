@@ -120,10 +120,12 @@ int dgk_mesh_build(int NeX, int NeY, int NeZ, int p, const double *x1D,
  * plane (XCD x walks the eighth x of every plane in turn; the FP64 VALU
  * kernels keep the element numbering where an eighth holds fewer than 96
  * elements), and consecutive stage launches walk the elements in opposite
- * directions.  Environment: DGK_ORDER=identity keeps the element numbering,
- * DGK_ALT=0 keeps one direction (DGK_ORDER=zperiod:Q deals eighths of every
- * Q elements instead, 8 <= Q, Q dividing Ne; DGK_ORDER=brick:bx,by,bz,
- * structured solvers only, walks bricks of elements instead). */
+ * directions (each XCD its own elements, in that order).  Environment:
+ * DGK_ORDER=identity keeps the element numbering, DGK_ALT=0 keeps one
+ * direction, DGK_ALT=global reverses the whole order instead
+ * (DGK_ORDER=zperiod:Q deals eighths of every Q elements instead, 8 <= Q, Q
+ * dividing Ne; DGK_ORDER=brick:bx,by,bz, structured solvers only, walks
+ * bricks of elements instead). */
 int dgk_create(int evaltype, int Nq, int64_t Ne, int64_t NeA,
                int64_t NhaloNode, const double *D1D, const double *Lift_mat,
                const int *VMapM, const int *VMapP, const int *halo_src_map,

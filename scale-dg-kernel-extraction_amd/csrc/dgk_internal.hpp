@@ -95,17 +95,20 @@ __device__ __forceinline__ void nbr_entries(const int32_t *__restrict__ vmapP,
   }
 }
 // Element of launch slot `slot` (< Ne): the processing order, walked
-// forwards or backwards.  Without an order array, zperiod = Q > 0 (8 <= Q,
-// Ne a multiple of Q) deals every Q consecutive elements of the numbering to
-// the 8 XCDs in eighths: with n = Ne / Q periods and o_x = floor(x Q / 8),
-// the slots [n o_x, n o_{x+1}), which the XCD x runs, are the elements
+// forwards (reverse = 0) or backwards (1: the whole order; 2: with a period,
+// each XCD's own slot range, so that an XCD walks the same elements in both
+// directions).  Without an order array, zperiod = Q > 0 (8 <= Q, Ne a
+// multiple of Q) deals every Q consecutive elements of the numbering to the
+// 8 XCDs in eighths: with n = Ne / Q periods and o_x = floor(x Q / 8), the
+// slots [n o_x, n o_{x+1}), which the XCD x runs, are the elements
 // [o_x, o_{x+1}) of every period, period after period.  Q = P (planes of P
 // elements) gives XCD x the eighth x of every plane; Q = 8 G deals groups of
 // G elements round-robin.
 __device__ __forceinline__ int64_t slot_element(int64_t slot, int64_t Ne,
                                                 const int32_t *__restrict__ order, int reverse,
                                                 int64_t zperiod) {
-  if (reverse) slot = Ne - 1 - slot;
+  const bool per_xcd = reverse == 2 && !order && zperiod > 0;
+  if (reverse && !per_xcd) slot = Ne - 1 - slot;
   if (order) return (int64_t)order[slot];
   if (zperiod > 0) {
     const uint32_t Q = (uint32_t)zperiod, n = (uint32_t)Ne / Q, s = (uint32_t)slot;
@@ -114,7 +117,9 @@ __device__ __forceinline__ int64_t slot_element(int64_t slot, int64_t Ne,
     for (uint32_t j = 1; j < 8; ++j) x += s >= n * (uint32_t)(((uint64_t)j * Q) >> 3) ? 1u : 0u;
     const uint32_t o = (uint32_t)(((uint64_t)x * Q) >> 3);
     const uint32_t g = (uint32_t)(((uint64_t)(x + 1) * Q) >> 3) - o;
-    const uint32_t t = s - n * o, k = t / g;
+    uint32_t t = s - n * o;
+    if (per_xcd) t = n * g - 1 - t;
+    const uint32_t k = t / g;
     return (int64_t)k * Q + o + (t - k * g);
   }
   return slot;

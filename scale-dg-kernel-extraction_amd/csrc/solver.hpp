@@ -36,7 +36,7 @@ struct dgk_solver {
   float *D32 = nullptr, *ell32 = nullptr;
   double *ell = nullptr;     // separable Lift (nullptr: stored-Lift kernel)
   int32_t *order = nullptr;  // element processing order
-  bool alternate = false;    // consecutive stage launches walk the order both ways
+  int alternate = 0;         // reverse mode of every other stage launch (slot_element)
   int64_t zperiod = 0;       // period of the XCD-interleaved order (slot_element)
   uint64_t launches = 0;     // stage launches so far (picks the direction)
   int axis_aligned = 0;
@@ -93,8 +93,9 @@ double *field_ptr(dgk_solver *s, int which);
 int refresh_f32(dgk_solver *s, int which);
 // One stage launch on `stream` (the solver's own stream when nullptr);
 // dt_geo takes the time step from each element's geometry record.
-// Consecutive stage launches alternate direction unless DGK_ALT=0.
-bool alternate_launches();
+// Consecutive stage launches alternate direction unless DGK_ALT=0: the
+// reverse mode of every other launch (slot_element).
+int alternate_launches();
 int launch_stage(dgk_solver *s, const double *qin, const double *q0, double *out,
                  int mode, double ca, double cb, double dt, hipStream_t stream = nullptr,
                  int dt_geo = 0);

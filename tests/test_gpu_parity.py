@@ -260,14 +260,17 @@ def test_run_kernel_uniform_tilted_normals(p):
 ORDER_MESHES = {
     # plane of 64 elements: the default period 64 gives each XCD one row of 8
     # per plane; zperiod:512 is whole planes, zperiod:128 two rows per turn
-    (8, 8, 16): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"},
+    # DGK_ALT=global walks the whole order backwards on every other launch
+    # (the default walks each XCD's own range backwards)
+    (8, 8, 16): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"}, {"DGK_ALT": "global"},
                  {"DGK_ORDER": "identity", "DGK_ALT": "0"}, {"DGK_ORDER": "brick:2,2,4"},
                  {"DGK_ORDER": "zperiod:512"}, {"DGK_ORDER": "zperiod:128"}),
     # plane of 24 elements, 5 planes: eighths of 3 elements; groups of 5
     (4, 6, 5): ({}, {"DGK_ORDER": "identity"}, {"DGK_ORDER": "zperiod:40"}),
     # planes of 15 and 25 elements: uneven eighths (1 or 2, 3 or 4 elements);
     # zperiod:120 deals whole planes of 15
-    (3, 5, 8): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"}, {"DGK_ORDER": "zperiod:120"}),
+    (3, 5, 8): ({}, {"DGK_ORDER": "identity"}, {"DGK_ALT": "0"}, {"DGK_ALT": "global"},
+                {"DGK_ORDER": "zperiod:120"}),
     (5, 5, 3): ({}, {"DGK_ORDER": "identity"}),
     # eighths of 128 elements: the FP64 VALU kernels take the interleaved
     # order too (below 96 elements per eighth they keep the numbering)
