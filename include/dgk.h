@@ -277,14 +277,21 @@ int dgk_batch_timer(dgk_batch *b, double *seconds, int64_t *calls);
  *           Defaults: the register VALU kernel at p = 4 and 6..13, the
  *           direction-by-direction MFMA shape (the last candidate) at
  *           p = 14..19, the staged VALU kernel otherwise.
- * dgk_tile_select sets the process-wide choice for that form and order
- * (-1 restores the default); it holds for every solver that has not been
- * autotuned.  Launches with a given flux (dgk_cal_dqdt) or a stored,
- * non-separable Lift always use the VALU kernels.  dgk_autotune times each
- * candidate with HIP events over reps RK-stage launches on the solver's own
- * mesh (writing only the solver's scratch rotation buffer) and keeps the
- * fastest for that solver alone, since the best shape depends on the mesh
- * (the default stays unless another shape is at least 2% faster);
+ * Constant-velocity launches (batch groups of constant-velocity instances,
+ * served requests) start from defaults of their own, taken only among shapes
+ * that give the bits of the defaults above: FP64 the register VALU kernel
+ * also at p = 2, 3; mixed two wavefronts at p = 3, one at p = 5, the
+ * direction-by-direction shape at p = 6, 9, 12.
+ * dgk_tile_select sets the process-wide choice for that form and order, for
+ * both kinds of launch (-1 restores the defaults); it holds for every solver
+ * that has not been autotuned; dgk_tile_selected reports the choice for
+ * launches with velocity fields.  Launches with a given flux (dgk_cal_dqdt)
+ * or a stored, non-separable Lift always use the VALU kernels.  dgk_autotune
+ * times each candidate with HIP events over reps RK-stage launches on the
+ * solver's own mesh (writing only the solver's scratch rotation buffer) and
+ * keeps the fastest for that solver alone, since the best shape depends on
+ * the mesh (the solver's default stays unless another shape is at least 2%
+ * faster);
  * us_per_launch[c] receives candidate c's time (0 for a candidate that would
  * launch another shape on this solver: the kernel over runs of elements, on
  * launches it cannot take -- constant velocities, a resolved face-node table,

@@ -1151,6 +1151,14 @@ int tile_selected(int evaltype, int Nq) {
   return evaltype == DGK_EVAL_HIP_MP ? stage_mp_selected(Nq) : stage_f64_selected(Nq);
 }
 
+// The process-wide selection that launches of s take: constant-velocity
+// launches (batches, served requests) have defaults of their own.
+int solver_selected(const dgk_solver *s) {
+  if (!s->cvel) return tile_selected(s->evaltype, s->Nq);
+  return s->evaltype == DGK_EVAL_HIP_MP ? stage_mp_selected_cvel(s->Nq)
+                                        : stage_f64_selected_cvel(s->Nq);
+}
+
 // Average device time of one stage launch of s, in microseconds.
 int time_stage(dgk_solver *s, int reps, double *us) {
   double *A = s->q[s->cur], *C = s->q[(s->cur + 2) % 3];
@@ -1205,7 +1213,7 @@ int autotune(dgk_solver *s, int reps, int *best, double *us, int max, int *n) {
   }
   if (arg < 0) return set_error(DGK_ERR_UNSUPPORTED, "no tile shape applies");
   // keep the default unless another shape is at least kTuneMargin faster
-  const int def = tile_selected(s->evaltype, s->Nq);
+  const int def = solver_selected(s);
   if (def >= 0 && def < nc && t[def] > 0.0 && t[arg] > (1.0 - kTuneMargin) * t[def]) arg = def;
   s->tile = arg;
   if (best) *best = arg;
@@ -1246,7 +1254,7 @@ int dgk_solver_tile(const dgk_solver *s, int *cand) {
   clear_error();
   DGK_TRY(check_solver(s));
   if (!cand) return set_error(DGK_ERR_ARG, "NULL argument");
-  *cand = s->tile >= 0 ? s->tile : tile_selected(s->evaltype, s->Nq);
+  *cand = s->tile >= 0 ? s->tile : solver_selected(s);
   return DGK_OK;
 }
 
@@ -1274,7 +1282,7 @@ int dgk_batch_autotune(dgk_batch *b, int reps, int *p, int *best, double *us_bes
       if (p) p[g] = b->group_p[g];
       if (best) best[g] = arg;
       if (us_best) us_best[g] = t[arg];
-      const int def = tile_selected(s->evaltype, s->Nq);
+      const int def = solver_selected(s);
       if (us_default) us_default[g] = t[def >= 0 && def < nc ? def : 0];  // 0: not applicable
     }
   }

@@ -296,11 +296,13 @@ int prepare_stage_valu_mp(int Nq);
 int stage_mp_shapes(int Nq, int *epb, int n);
 int stage_mp_select(int Nq, int cand);
 int stage_mp_selected(int Nq);
+int stage_mp_selected_cvel(int Nq);   // the selection for constant-velocity launches
 // FP64: candidates 0, 1 are the VALU kernels, 2.. the FP64 MFMA kernel's shapes.
 constexpr int kMaxTileShapes = 8;
 int stage_f64_shapes(int Nq, int *epb, int n);
 int stage_f64_select(int Nq, int cand);
 int stage_f64_selected(int Nq);
+int stage_f64_selected_cvel(int Nq);  // the selection for constant-velocity launches
 // FP64 MFMA stage kernel (kernels_mfma.hip): shapes, attributes, launch of
 // shape candidate `cand`.
 int tile_f64_shapes(int Nq, int *epb, int n);

@@ -866,6 +866,15 @@ constexpr int kF64Default[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 
 // serving worker may launch while another thread selects.
 std::atomic<int> g_f64_shape[kMaxNq + 1] = {0, 0, 0, 0, 0, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1,
                                             3, 3, 1, 1, 0, 0};
+// Defaults of constant-velocity launches (batches, served requests), from the
+// same timings with the velocity read from the geometry record: the register
+// VALU kernel also at p = 2, 3 (185 -> 165 and 155 -> 140 us per launch).
+// Only shapes that give the bits of the default above are taken, so a served
+// request equals its solve on a plain solver.
+constexpr int kF64DefaultCvel[kMaxNq + 1] = {0, 0, 0, 1, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
+                                             1, 1, 0, 0};
+std::atomic<int> g_f64_shape_cvel[kMaxNq + 1] = {0, 0, 0, 1, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1,
+                                                 3, 3, 1, 1, 0, 0};
 
 bool stage_f64_candidate_runs(int Nq, int cand, const StageArgs &a) {
   if (Nq < kMinNq || Nq > kMaxNqValu) return true;
@@ -877,8 +886,9 @@ bool stage_f64_candidate_runs(int Nq, int cand, const StageArgs &a) {
 int launch_stage_f64(int Nq, const StageArgs &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
   int cand = (Nq < kMinNq || Nq > kMaxNq) ? 0
-                   : a.tile >= 0                ? a.tile
-                                                : g_f64_shape[Nq].load(std::memory_order_relaxed);
+             : a.tile >= 0                ? a.tile
+                                          : (a.cvel ? g_f64_shape_cvel : g_f64_shape)[Nq].load(
+                                                std::memory_order_relaxed);
   if (Nq > kMaxNqValu && Nq <= kMaxNq) {
     if (a.flux_in || !a.ell)
       return set_error(DGK_ERR_UNSUPPORTED, "above p = 15 the stage runs on the MFMA kernel "
@@ -985,15 +995,19 @@ int stage_f64_select(int Nq, int cand) {
   int e[kMaxTileShapes];
   const int nc = stage_f64_shapes(Nq, e, kMaxTileShapes);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
-  if (cand == -1) cand = kF64Default[Nq];
-  if (cand < 0 || cand >= nc)
+  if (cand < -1 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
-  g_f64_shape[Nq].store(cand, std::memory_order_relaxed);
+  g_f64_shape[Nq].store(cand == -1 ? kF64Default[Nq] : cand, std::memory_order_relaxed);
+  g_f64_shape_cvel[Nq].store(cand == -1 ? kF64DefaultCvel[Nq] : cand, std::memory_order_relaxed);
   return DGK_OK;
 }
 
 int stage_f64_selected(int Nq) {
   return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_f64_shape[Nq].load(std::memory_order_relaxed);
+}
+
+int stage_f64_selected_cvel(int Nq) {
+  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_f64_shape_cvel[Nq].load(std::memory_order_relaxed);
 }
 
 int launch_elembnd_flux_f64(int Nq, const double *q, const double *u,

@@ -1002,6 +1002,17 @@ constexpr int kMpDefault[kMaxNq + 1] = {0, 0, 5, 0, 0, 1, 2, 0, 0, 0, 0,
                                         0, 0, 0, 0, 2, 0, 1, 1, 0, 0};
 std::atomic<int> g_mp_shape[kMaxNq + 1] = {0, 0, 5, 0, 0, 1, 2, 0, 0, 0, 0,
                                            0, 0, 0, 0, 2, 0, 1, 1, 0, 0};
+// Defaults of constant-velocity launches (batches, served requests), from the
+// same timings with the velocity read from the geometry record, among the
+// shapes that give the bits of the default above (the MFMA shapes and the
+// run kernel), so a served request equals its solve on a plain solver: two
+// wavefronts at p = 3 (161 -> 152 us), one wavefront at p = 5 (119 -> 105 us),
+// the direction-by-direction shape at p = 6, 9, 12 (109 -> 102, 105 -> 98,
+// 112 -> 106 us).
+constexpr int kMpDefaultCvel[kMaxNq + 1] = {0, 0, 5, 0, 1, 1, 0, 3, 0, 0, 2,
+                                            0, 0, 2, 0, 2, 0, 1, 1, 0, 0};
+std::atomic<int> g_mp_shape_cvel[kMaxNq + 1] = {0, 0, 5, 0, 1, 1, 0, 3, 0, 0, 2,
+                                                0, 0, 2, 0, 2, 0, 1, 1, 0, 0};
 
 template <class P, int NQ>
 int prepare_nq_all() {
@@ -1080,7 +1091,8 @@ bool stage_mp_candidate_runs(int Nq, int cand, const StageArgsMP &a) {
 int launch_stage_mp(int Nq, const StageArgsMP &a, hipStream_t s) {
   if (a.Ne < 1) return DGK_OK;
   int cand = a.tile;
-  if (cand < 0 && Nq >= kMinNq && Nq <= kMaxNq) cand = g_mp_shape[Nq].load(std::memory_order_relaxed);
+  if (cand < 0 && Nq >= kMinNq && Nq <= kMaxNq)
+    cand = (a.cvel ? g_mp_shape_cvel : g_mp_shape)[Nq].load(std::memory_order_relaxed);
   // the VALU kernel (without the FP64 operators, the first shape runs) and the
   // run kernel (launches it cannot take run the first shape, with the same bits)
   if (cand >= 0 && cand == mp_valu_index(Nq)) {
@@ -1134,15 +1146,19 @@ int stage_mp_select(int Nq, int cand) {
   int e[kMaxTileShapes];
   const int nc = stage_mp_shapes(Nq, e, kMaxTileShapes);
   if (nc < 1) return set_error(DGK_ERR_UNSUPPORTED, "Nq outside 2.." + std::to_string(kMaxNq));
-  if (cand == -1) cand = kMpDefault[Nq];
-  if (cand < 0 || cand >= nc)
+  if (cand < -1 || cand >= nc)
     return set_error(DGK_ERR_ARG, "tile-shape candidate outside 0.." + std::to_string(nc - 1));
-  g_mp_shape[Nq].store(cand, std::memory_order_relaxed);
+  g_mp_shape[Nq].store(cand == -1 ? kMpDefault[Nq] : cand, std::memory_order_relaxed);
+  g_mp_shape_cvel[Nq].store(cand == -1 ? kMpDefaultCvel[Nq] : cand, std::memory_order_relaxed);
   return DGK_OK;
 }
 
 int stage_mp_selected(int Nq) {
   return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_mp_shape[Nq].load(std::memory_order_relaxed);
+}
+
+int stage_mp_selected_cvel(int Nq) {
+  return (Nq < kMinNq || Nq > kMaxNq) ? 0 : g_mp_shape_cvel[Nq].load(std::memory_order_relaxed);
 }
 
 #undef DGK_NQ_SWITCH

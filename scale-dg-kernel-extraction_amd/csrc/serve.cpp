@@ -309,7 +309,7 @@ int start(dgk_server *sv, Req *r, int64_t off) {
   s.axis_aligned = 1;
   // the tile shape is fixed at admission, so a later dgk_tile_select from
   // another thread never changes a request in flight
-  s.tile = mp ? stage_mp_selected(r->Nq) : stage_f64_selected(r->Nq);
+  s.tile = mp ? stage_mp_selected_cvel(r->Nq) : stage_f64_selected_cvel(r->Nq);
   s.alternate = alternate_launches();
   s.launches = 0;
   // the XCD-interleaved order of plane-numbered meshes (box_nbr numbers

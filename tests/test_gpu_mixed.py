@@ -203,11 +203,10 @@ def test_autotune_solver_and_batch():
         assert sorted(x["p"] for x in rec) == sorted({it["p"] for it in ens})
         for x in rec:
             assert x["epb"] in dgk.tile_shapes("HIP_MP", x["p"]) and x["us_per_launch"] > 0
-            # the default's time is the default candidate's own timing
-            if x["candidate"] == dgk.tile_selected("HIP_MP", x["p"]):
-                assert x["us_default"] == x["us_per_launch"]
-            else:
-                assert x["us_per_launch"] <= 0.98 * x["us_default"]
+            # the group's default (its constant-velocity default) stays unless
+            # another candidate is at least 2% faster
+            assert (x["us_per_launch"] == x["us_default"]
+                    or x["us_per_launch"] <= 0.98 * x["us_default"])
         b.step(1)  # re-captured with the tuned shapes
         valu = {x["p"] for x in rec if x["candidate"] not in mfma_candidates(x["p"])}
         for k, it in enumerate(ens):
